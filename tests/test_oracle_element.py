@@ -1,0 +1,193 @@
+"""Pins the oracle's element operators and physics:
+the reference's own element identities (test_TensorQuad.cpp:71-206,
+test_TensorHex.cpp:65-240, test_source.cpp), plus closed forms that a wrong
+index, sign or material mapping would break."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from salvus_amd import meshgen
+
+QUAD_REF = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]], float)
+HEX_REF = np.array([[-1, -1, -1], [-1, 1, -1], [1, 1, -1], [1, -1, -1],
+                    [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], float)
+
+
+@pytest.mark.parametrize("dim,order", [(2, o) for o in range(1, 11)] + [(3, o) for o in range(1, 8)])
+def test_basis_gradients(dim, order):
+    """Gradient of every basis function at every GLL point equals the
+    analytic Lagrange derivative (reference element, J = I)."""
+    n = order + 1
+    nd = n ** dim
+    vtx = QUAD_REF if dim == 2 else HEX_REF
+    Dm = po.deriv_matrix(order)
+    grads = np.stack([po.elem_gradient(dim, order, vtx, np.eye(nd)[q]) for q in range(nd)])
+    idx = np.indices((n,) * dim).reshape(dim, -1)[::-1]  # (i,j[,k]) per node
+    for a in range(dim):
+        expect = np.ones((nd, nd))
+        for b in range(dim):
+            if a == b:
+                expect *= Dm[idx[b][None, :], idx[b][:, None]]  # [q, p] -> D[p_b, q_b]
+            else:
+                expect *= (idx[b][None, :] == idx[b][:, None])
+        np.testing.assert_allclose(grads[:, :, a], expect, atol=1e-10)
+
+
+@pytest.mark.parametrize("dim,order", [(2, 3), (2, 4), (3, 3), (3, 4), (3, 5)])
+def test_integration_identities(dim, order):
+    nd = (order + 1) ** dim
+    vtx = QUAD_REF if dim == 2 else HEX_REF
+    for p in range(nd):
+        f = np.zeros((nd, dim))
+        f[p, :] = 1.0
+        # sum_i int grad(phi_i) . F = 0  (test_TensorQuad.cpp:145-146)
+        assert abs(po.elem_grad_test_integrate(dim, order, vtx, f).sum()) < 1e-12
+    # delta function integrates to one (test_TensorQuad.cpp:181-189)
+    coeff = po.elem_delta_coeffs(dim, order, vtx, np.zeros(dim))
+    assert po.elem_test_integrate(dim, order, vtx, coeff).sum() == pytest.approx(1.0)
+    # parameters interpolate (test_TensorQuad.cpp:198-202)
+    assert po.elem_par_at_int_pts(dim, order, np.ones(2 ** dim)).sum() == pytest.approx(nd)
+
+
+def _sigma(dim, physics, par, A):
+    """Textbook stress of the linear field u = A x for the reference's
+    material parameters (Elastic2D.cpp:53-82 with C12=C23=0; Elastic3D.cpp:
+    33-54 transversely isotropic)."""
+    if physics == po.ACOUSTIC:
+        return par[0] ** 2 * A[0]
+    if dim == 2:
+        rho, c11, c13, c33, c55 = par
+        exx, eyy, g = A[0, 0], A[1, 1], A[0, 1] + A[1, 0]
+        return np.array([[c11 * exx + c13 * eyy, c55 * g], [c55 * g, c13 * exx + c33 * eyy]])
+    rho, vpv, vph, vsv, vsh, eta = par
+    c11 = c22 = rho * vph ** 2
+    c33, c44, c66 = rho * vpv ** 2, rho * vsv ** 2, rho * vsh ** 2
+    c55, c12, c13 = c44, c11 - 2 * c66, eta * (c11 - 2 * c44)
+    c23 = c13
+    e = [A[0, 0], A[1, 1], A[2, 2]]
+    syz, sxz, sxy = c44 * (A[1, 2] + A[2, 1]), c55 * (A[0, 2] + A[2, 0]), c66 * (A[0, 1] + A[1, 0])
+    sxx = c11 * e[0] + c12 * e[1] + c13 * e[2]
+    syy = c12 * e[0] + c22 * e[1] + c23 * e[2]
+    szz = c13 * e[0] + c23 * e[1] + c33 * e[2]
+    return np.array([[sxx, sxy, sxz], [sxy, syy, syz], [sxz, syz, szz]])
+
+
+@pytest.mark.parametrize("dim,physics", [(2, po.ACOUSTIC), (3, po.ACOUSTIC), (2, po.ELASTIC),
+                                         (3, po.ELASTIC)])
+def test_linear_field_traction(dim, physics):
+    """For u = A x, K_e u vanishes at interior nodes and equals sigma.n
+    times the face quadrature weight at face nodes (Gauss divergence theorem,
+    exact under GLL quadrature for a box element)."""
+    order = 4
+    n = order + 1
+    h = np.array([2.0, 3.0, 5.0])[:dim]
+    vtx = (QUAD_REF if dim == 2 else HEX_REF) * h / 2 + h / 2
+    rng = np.random.default_rng(7)
+    c = 1 if physics == po.ACOUSTIC else dim
+    if physics == po.ACOUSTIC:
+        par = np.array([1700.0])
+    elif dim == 2:
+        par = np.array([2500.0, 2.1e10, 0.7e10, 1.8e10, 0.6e10])
+    else:
+        par = np.array([2600.0, 5000.0, 5600.0, 2900.0, 3300.0, 0.83])
+    A = rng.normal(size=(c, dim))
+    sigma = np.atleast_2d(_sigma(dim, physics, par, A))
+    xyz = po.nodal_points(order, vtx)
+    u = (xyz @ A.T).T  # c x nd
+    ku = po.elem_stiffness(dim, order, physics, vtx, np.tile(par, 2 ** dim), u).reshape(c, -1)
+    _, w = po.gll(order)
+    idx = np.indices((n,) * dim).reshape(dim, -1)[::-1]
+    scale = np.abs(ku).max()
+    for p in range(n ** dim):
+        ip = idx[:, p]
+        onb = (ip == 0) | (ip == order)
+        if onb.sum() == 0:
+            np.testing.assert_allclose(ku[:, p], 0, atol=1e-9 * scale)
+        elif onb.sum() == 1:
+            a = int(np.nonzero(onb)[0][0])
+            sgn = 1.0 if ip[a] == order else -1.0
+            wf = np.prod([w[ip[b]] * h[b] / 2 for b in range(dim) if b != a])
+            np.testing.assert_allclose(ku[:, p], sgn * sigma[:, a] * wf, rtol=1e-9,
+                                       atol=1e-9 * scale)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_rigid_modes_and_symmetry(dim):
+    order = 3
+    if dim == 2:
+        cells, coords = meshgen.box_quad(3, 2, 3.0, 2.0)
+        par = np.tile([2500.0, 2e10, 0.5e10, 1.5e10, 0.7e10], 4 * len(cells))
+    else:
+        cells, coords = meshgen.box_hex(2, 2, 2, 2.0, 2.0, 2.0)
+        par = np.tile([2500.0, 5000.0, 5200.0, 2800.0, 3000.0, 0.9], 8 * len(cells))
+    S = po.Sim(dim, order, po.ELASTIC, cells, coords, par)
+    xyz = S.node_coords()
+    rot = np.zeros((dim, S.G))
+    rot[0], rot[1] = -xyz[:, 1], xyz[:, 0]  # infinitesimal rotation about z
+    for u in (np.ones((dim, S.G)), rot):
+        assert np.abs(S.apply_stiffness(u)).max() < 1e-6 * 1e10
+    rng = np.random.default_rng(0)
+    x, y = rng.normal(size=(2, dim, S.G))
+    kx, ky = S.apply_stiffness(x), S.apply_stiffness(y)
+    assert np.vdot(kx, y) == pytest.approx(np.vdot(x, ky), rel=1e-10)
+    assert np.vdot(kx, x) > 0
+
+
+def test_ricker_closed_form():  # test_source.cpp:18-22, :263-291
+    for amp, t0, f0 in ((10, 0.1, 50), (20, 0.01, 60)):
+        for j in range(0, 1000, 37):
+            t = j * 1e-3
+            fac = np.pi * np.pi * f0 * f0 * (t - t0) * (t - t0)
+            assert po.ricker(t, amp, f0, t0) == amp * ((1 - 2 * fac) * math.exp(-1 * fac))
+
+
+@pytest.mark.parametrize("dim,physics", [(2, po.ACOUSTIC), (2, po.ELASTIC), (3, po.ACOUSTIC)])
+def test_source_term_sums_to_amplitude(dim, physics):
+    """test_source.cpp:293-374: the owning element's nodal load sums to the
+    amplitude; every other element gets nothing."""
+    order = 3
+    if dim == 2:
+        cells, coords = meshgen.box_quad(3, 3, 3.0, 3.0)
+    else:
+        cells, coords = meshgen.box_hex(2, 2, 2, 2.0, 2.0, 2.0)
+    npar = 1 if physics == po.ACOUSTIC else 5
+    S = po.Sim(dim, order, physics, cells, coords, np.ones(len(cells) * 2 ** dim * npar))
+    loc = np.array([1.3, 0.4, 0.7])[:dim]
+    k = S.add_source(loc, 10.0, 1.0, 0.0, [1.0] * S.c)
+    elem, ref = S.source_info(k)
+    assert po.check_hull(loc, coords[cells[elem]]) == 1
+    assert all(po.check_hull(loc, coords[cells[e]]) == 0 for e in range(elem))
+    load = S.source_load(k)
+    for comp in range(S.c):
+        assert load[comp].sum() == pytest.approx(1.0)  # times amplitude * R(t) at run time
+
+
+def test_elastic3d_source_quirk():
+    """Q7: Elastic3D's load is l_i(xs)/(w_i detJ_i), Scalar's l_i(xs)."""
+    cells, coords = meshgen.box_hex(1, 1, 1, 2.0, 2.0, 2.0)
+    par = np.tile([1.0, 2.0, 2.0, 1.0, 1.0, 1.0], 8)
+    loc = [0.7, 1.1, 0.9]
+    a = po.Sim(3, 3, po.ELASTIC, cells, coords, par, quirks=po.QUIRK_E3D_SOURCE)
+    b = po.Sim(3, 3, po.ELASTIC, cells, coords, par, quirks=0)
+    a.add_source(loc, 1.0, 1.0, 0.0, [1.0, 0, 0])
+    b.add_source(loc, 1.0, 1.0, 0.0, [1.0, 0, 0])
+    la, lb = a.source_load(0)[0], b.source_load(0)[0]
+    _, w = po.gll(3)
+    wq = np.einsum("k,j,i->kji", w, w, w).ravel()
+    np.testing.assert_allclose(la * wq * 1.0, lb, rtol=1e-12, atol=1e-15)  # detJ = 1
+
+
+def test_moment_tensor_load_moments():
+    """f_p = -M grad(phi_p)(xs): zero net force, first moment sum_p x_p f_p^T = -M^T."""
+    cells, coords = meshgen.box_hex(2, 2, 2, 30.0, 30.0, 30.0)
+    par = np.tile([2500.0, 5000.0, 5000.0, 2900.0, 2900.0, 1.0], 8 * len(cells))
+    S = po.Sim(3, 4, po.ELASTIC, cells, coords, par)
+    M = np.array([[1.0, 0.3, -0.2], [0.7, -0.5, 0.4], [0.1, 0.9, 2.0]])
+    k = S.add_moment_source([11.0, 17.0, 9.5], M, 1.0, 1.0, 0.0)
+    elem, _ = S.source_info(k)
+    load = S.source_load(k)  # 3 x nd
+    x = S.node_coords()[S.conn()[elem]]
+    np.testing.assert_allclose(load.sum(axis=1), 0, atol=1e-12)
+    np.testing.assert_allclose(load @ x, -M, atol=1e-11)

@@ -1,0 +1,135 @@
+"""Pins the oracle's whole time loop
+against the reference's end-to-end eigenfunction tests
+(test_Scalar_2D.cpp:238-308, test_Scalar_3D.cpp:152-235), closed-form
+elastic standing waves, and the exact algebra of its Newmark scheme."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from salvus_amd import meshgen
+from tests import refmeshes as rm
+
+L, VP = 1e5, 5800.0  # test_Model.cpp:35; eigenfunction meshes span [0, 1e5]^d
+
+
+def eigen_run(dim, cells, coords, order, cycle, dt=1e-2):
+    nv = 2 ** dim
+    S = po.Sim(dim, order, po.ACOUSTIC, cells, coords, np.full(len(cells) * nv, VP))
+    xyz = S.node_coords()
+    # x0 = 5e4, x0 + L/2 = 1e5 (test_Scalar_3D.cpp:31-37)
+    un = np.prod(np.sin(np.pi / L * (xyz - (5e4 + L / 2))), axis=1)
+    S.set_field("u", un)
+    S.set_dirichlet(np.any(np.isclose(xyz, 0, atol=1e-6 * L) | np.isclose(xyz, L, atol=1e-6 * L),
+                           axis=1))
+    worst, t = 0.0, 0.0
+    while True:  # test_Scalar_3D.cpp:104-145
+        S.run(dt, 1)
+        t = S.time()
+        exact = np.cos(np.pi / L * np.sqrt(dim) * t * VP) * un
+        worst = max(worst, np.abs(exact - S.get_field("u")[0]).max())
+        if t > cycle:
+            break
+    return worst
+
+
+def test_quad_eigenfunction_reference_bound():
+    cells, coords = rm.quad_4x4()  # quad_eigenfunction.e: 4x4 quads of 25 km
+    err = eigen_run(2, cells, coords, 3, 24.39)
+    assert err <= 0.001288 * (1 + 0.01)  # test_Scalar_2D.cpp:305-306
+
+
+@pytest.mark.parametrize("order,bound", [(3, 0.00048205), (4, 0.000489815), (5, 0.000486752)])
+def test_hex_eigenfunction_reference_bound(order, bound):
+    # hex_eigenfunction.e is not available; a 3x3x3 box of the same extent is
+    # used.  The error is dominated by the scheme's start-up (a_ = 0 on the
+    # first step), whose amplitude is omega dt / 2 * sin(omega t).
+    cells, coords = meshgen.box_hex(3, 3, 3, L, L, L)
+    err = eigen_run(3, cells, coords, order, 1.0)
+    assert err <= bound * 1.01  # test_Scalar_3D.cpp:213-232
+    omega = np.pi / L * np.sqrt(3) * VP
+    assert err == pytest.approx(omega * 1e-2 / 2 * np.sin(omega * 1.0), rel=0.08)
+
+
+def test_newmark_is_central_difference():
+    """Order2Newmark.cpp:85-115 gives u_{n+1} - 2u_n + u_{n-1} = dt^2 a_n."""
+    cells, coords = meshgen.box_quad(3, 3, 3.0, 3.0)
+    S = po.Sim(2, 3, po.ACOUSTIC, cells, coords, np.full(len(cells) * 4, 2.0))
+    rng = np.random.default_rng(1)
+    S.set_field("u", rng.normal(size=S.G))
+    mi, dt = S.inverse_mass(), 1e-3
+    us = [S.get_field("u")[0].copy()]
+    for _ in range(4):
+        S.run(dt, 1)
+        us.append(S.get_field("u")[0].copy())
+    for n in (1, 2, 3):
+        a_n = -mi * S.apply_stiffness(us[n])[0]
+        np.testing.assert_allclose(us[n + 1] - 2 * us[n] + us[n - 1], dt * dt * a_n,
+                                   rtol=1e-8, atol=1e-12)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_elastic_standing_p_wave(dim):
+    """u_x = sin(pi x/L) cos(omega t) with Dirichlet x faces is exact when
+    the coupling stiffness C13 (2D) / c12, c13 (3D) vanishes."""
+    Lb = 1000.0
+    order, n_el = 4, 3
+    if dim == 2:
+        rho, c11 = 2000.0, 2000.0 * 3000.0 ** 2
+        cells, coords = meshgen.box_quad(n_el, n_el, Lb, Lb)
+        par = np.tile([rho, c11, 0.0, 1.1 * c11, 0.3 * c11], 4 * len(cells))
+    else:
+        rho, vph = 2000.0, 3000.0
+        c11 = rho * vph ** 2
+        # c12 = c11 - 2 c66 = 0 (VSH^2 = VPH^2/2); c13 = ETA (c11 - 2 c44) = 0 (ETA = 0)
+        cells, coords = meshgen.box_hex(n_el, n_el, n_el, Lb, Lb, Lb)
+        par = np.tile([rho, 3300.0, vph, 1400.0, vph / np.sqrt(2), 0.0], 8 * len(cells))
+    S = po.Sim(dim, order, po.ELASTIC, cells, coords, par)
+    xyz = S.node_coords()
+    ux = np.sin(np.pi * xyz[:, 0] / Lb)
+    u0 = np.zeros((dim, S.G))
+    u0[0] = ux
+    S.set_field("u", u0)
+    S.set_dirichlet(np.isclose(xyz[:, 0], 0, atol=1e-9 * Lb) | np.isclose(xyz[:, 0], Lb, atol=1e-9 * Lb))
+    omega, dt = np.pi / Lb * np.sqrt(c11 / rho), 2e-4
+    worst = 0.0
+    for _ in range(60):
+        S.run(dt, 10)
+        u = S.get_field("u")
+        worst = max(worst, np.abs(u[0] - np.cos(omega * S.time()) * ux).max())
+        assert np.abs(u[1:]).max() < 1e-9
+    # start-up amplitude omega dt / 2 bounds the error
+    assert worst < omega * dt / 2 * 1.05
+
+
+def test_step_count_quirk():
+    """Q2: dt = duration/ceil(duration/dt) and while(t < duration) with t
+    accumulated in FP64 (Options.cpp:84-97, Main.cpp:45)."""
+    assert po.reference_step_count(0.2, 2e-4)[0] == 1000  # configuration 1
+    n, dt = po.reference_step_count(1.0, 2e-4)  # configuration 2 duration
+    assert n == 5001 and dt == 1.0 / 5000
+    for duration, dt in ((1.0, 1e-2), (0.3, 1e-3), (2.5, 7e-4)):
+        m = int(np.ceil(duration / dt))
+        h, t, count = duration / m, 0.0, 0
+        while t < duration:
+            t += h
+            count += 1
+        assert po.reference_step_count(duration, dt)[0] == count
+
+
+def test_newmark3d_quirk():
+    """Order2Newmark.cpp:93-96 lists {ax, ay, ax, a}: with the quirk on, uz
+    never moves and ux takes two updates per step."""
+    cells, coords = meshgen.box_hex(2, 2, 2, 100.0, 100.0, 100.0)
+    par = np.tile([2000.0, 3000.0, 3000.0, 1700.0, 1700.0, 1.0], 8 * len(cells))
+    q = po.Sim(3, 3, po.ELASTIC, cells, coords, par, quirks=po.QUIRK_NEWMARK3D)
+    p = po.Sim(3, 3, po.ELASTIC, cells, coords, par, quirks=0)
+    rng = np.random.default_rng(2)
+    u0 = rng.normal(size=(3, q.G)) * 1e-3
+    for s in (q, p):
+        s.set_field("u", u0)
+    q.run(1e-4, 3)
+    p.run(1e-4, 3)
+    uq, up = q.get_field("u"), p.get_field("u")
+    np.testing.assert_array_equal(uq[2], u0[2])
+    assert np.abs(up[2] - u0[2]).max() > 0
+    assert np.abs(uq[0] - up[0]).max() > 0
