@@ -1,0 +1,149 @@
+/*
+ * salvus_sem.h -- C ABI of the MI355X spectral-element time-stepping path
+ * (tensor quads/hexes; Scalar, Elastic2D, Elastic3D physics).
+ *
+ * This is the drop-in boundary between the reference's C++ Problem/Newmark
+ * loop and the device.  Plain pointers and sizes only; every entry point
+ * returns 0 on success and a negative status otherwise, with the message in
+ * sem_last_error() (the reference throws std::runtime_error instead,
+ * Main.cpp:70-74).  The caller owns host buffers; the library owns device
+ * buffers.  A mesh or batch is bound to one device and one HIP stream and is
+ * driven from one host thread at a time.
+ *
+ * Conventions (same as the reference):
+ *   order N, n = N+1, nd = n^dim nodes per element, node p = i + n j (+ n^2 k)
+ *   with i along r (TensorQuad.cpp:187, Hexahedra.cpp:591);
+ *   cells in the vertex order of QuadP1.h:98-108 / HexP1.cpp:123-133;
+ *   vertex material per element: par[(e*nv + v)*npar + q] with parameters
+ *   Scalar {VP}, Elastic2D {RHO,C11,C13,C33,C55},
+ *   Elastic3D {RHO,VPV,VPH,VSV,VSH,ETA};
+ *   global fields are component-planar: f[comp*G + g] in the DMPlex DOF
+ *   numbering (cells, vertices, faces, edges; Mesh.cpp:361-452).
+ */
+#ifndef SALVUS_SEM_H
+#define SALVUS_SEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sem_mesh sem_mesh;
+typedef struct sem_batch sem_batch;
+typedef struct sem_server sem_server;
+
+enum { SEM_ACOUSTIC = 0, SEM_ELASTIC = 1 };
+
+/* Reference-behaviour switches, see DESIGN: Q7 and Q20. */
+enum { SEM_QUIRK_NEWMARK3D = 1, SEM_QUIRK_E3D_SOURCE = 2 };
+#define SEM_DEFAULT_QUIRKS SEM_QUIRK_E3D_SOURCE
+
+/* Execution paths of a batch. */
+enum {
+  SEM_PATH_PLAIN = 0, /* one shot per launch: gather, stiffness, colour scatter, update */
+  SEM_PATH_FUSED = 1  /* fused colour kernels over all shots, hipGraph step blocks */
+};
+
+/* Field selectors: displacement, velocity, previous acceleration (a_). */
+enum { SEM_FIELD_U = 0, SEM_FIELD_V = 1, SEM_FIELD_A_PREV = 2 };
+
+/* Thread-local text of the last error (replaces the reference's exception
+ * messages, Main.cpp:70-74, Logging.cpp:9-12). */
+const char *sem_last_error(void);
+/* Number of visible HIP devices. */
+int sem_device_count(int *count);
+
+/* Host-only gather map and colouring (no device needed): the closure
+ * numbering of Mesh::setupGlobalDof (Mesh.cpp:361-452) and the element
+ * colouring used by the scatter.  conn: E x nd (or NULL), colour: E (or
+ * NULL); info[0..3] = {G unique nodes, colours, faces, edges}. */
+int sem_topology_build(int dim, int order, int64_t num_cells, int64_t num_verts,
+                       const int32_t *cells, int32_t *conn, int32_t *colour, int64_t *info);
+
+/* ------------------------------------------------------------------ mesh
+ * Replaces Mesh::read(dim, numCells, numVerts, numVertsPerElem, cells,
+ * coords) (Mesh.cpp:31-51), Mesh::setupGlobalDof (Mesh.cpp:361-452),
+ * Element::attachVertexCoordinates / attachMaterialProperties
+ * (TensorQuad.cpp:212-251, Hexahedra.cpp:132-162, :414-425,
+ * Elastic3D.cpp:33-54) and Order2Newmark::initializeGlobalDofs' mass
+ * assembly (Order2Newmark.cpp:32-66).  Builds the gather map, element
+ * colouring and FP64 geometry/mass on the device. */
+int sem_mesh_create(sem_mesh **mesh, int device, int dim, int order, int physics,
+                    int64_t num_cells, int64_t num_verts, const int32_t *cells,
+                    const double *coords, const double *vertex_par, int quirks);
+int sem_mesh_destroy(sem_mesh *mesh);
+/* info[0..7] = {E, G (unique nodes), ncolours, components, nodes/element,
+ *               affine (1/0), device bytes held, dim} */
+int sem_mesh_info(const sem_mesh *mesh, int64_t *info);
+/* Gather map conn[e*nd + p] = global DOF, element order as given
+ * (DMPlexVecGetClosure + ClsMap, Problem.cpp:302-320). */
+int sem_mesh_get_conn(const sem_mesh *mesh, int32_t *conn);
+/* Element colours (no two elements of one colour share a node). */
+int sem_mesh_get_colours(const sem_mesh *mesh, int32_t *colour);
+/* Assembled inverse diagonal mass, FP32 (Order2Newmark.cpp:52). */
+int sem_mesh_get_inverse_mass(const sem_mesh *mesh, float *mi);
+/* Global node coordinates (G x dim, FP64). */
+int sem_mesh_get_node_coords(const sem_mesh *mesh, double *xyz);
+/* Homogeneous Dirichlet: zero stiffness rows at masked nodes
+ * (HomogeneousDirichlet.cpp:71-78). */
+int sem_mesh_set_dirichlet(sem_mesh *mesh, const uint8_t *node_mask);
+/* One assembled stiffness application ku = K u (c x G, FP32) on the device
+ * through the given path's kernels (Problem.cpp:176-208 without source and
+ * mass); u and ku are host buffers. */
+int sem_mesh_apply_stiffness(sem_mesh *mesh, int path, const float *u, float *ku);
+/* Race check on the device: returns in *violations the number of
+ * (colour, node) pairs touched by more than one element of that colour. */
+int sem_mesh_check_colouring(sem_mesh *mesh, int64_t *violations);
+
+/* ----------------------------------------------------------------- shots
+ * A batch holds shots (sources + receivers + wavefields) on one or more
+ * meshes and advances them together.  Replaces Problem::initializeElements'
+ * source/receiver attachment (Problem.cpp:34-142), the time loop
+ * (Main.cpp:45-66) with assembleIntoGlobalDof / applyInverseMassMatrix /
+ * takeTimeStep (Problem.cpp:143-215, Order2Newmark.cpp:68-115). */
+int sem_batch_create(sem_batch **batch, int device);
+int sem_batch_destroy(sem_batch *batch);
+/* Adds a shot on `mesh` that will advance nsteps steps of dt; *shot gets
+ * its index in the batch. */
+int sem_batch_add_shot(sem_batch *batch, sem_mesh *mesh, int64_t nsteps, double dt, int *shot);
+/* Ricker point source (Ricker.cpp:27-33) at loc, owned by the first element
+ * whose closed hull holds it (Problem.cpp:57-113); dir has c entries. */
+int sem_shot_add_source(sem_batch *batch, int shot, const double *loc, double amp, double f0,
+                        double t0, const double *dir);
+/* Moment-tensor Ricker source (no reference counterpart):
+ * f_p = -M grad(phi_p)(loc) R(t); M is dim x dim row-major. */
+int sem_shot_add_moment_source(sem_batch *batch, int shot, const double *loc, const double *M,
+                               double amp, double f0, double t0);
+/* Receiver sampling u at loc after every step (the reference locates
+ * receivers but never records, Receiver.cpp:48-59). */
+int sem_shot_add_receiver(sem_batch *batch, int shot, const double *loc);
+/* Initial/current fields (c x G, FP64 host values stored FP32). */
+int sem_shot_set_field(sem_batch *batch, int shot, int which, const double *vals);
+int sem_shot_get_field(sem_batch *batch, int shot, int which, double *vals);
+/* Allocates device state and records the step plan.  path: SEM_PATH_*;
+ * elems_per_block / shots_per_group: 0 = autotuned. */
+int sem_batch_prepare(sem_batch *batch, int path, int elems_per_block, int shots_per_group);
+/* Advances every unfinished shot by up to nsteps steps. */
+int sem_batch_run(sem_batch *batch, int64_t nsteps);
+/* Steps taken so far by a shot and its simulated time. */
+int sem_shot_progress(sem_batch *batch, int shot, int64_t *steps, double *time);
+/* Receiver samples nsteps x R x c (FP32), sampled after each step. */
+int sem_shot_get_seismograms(sem_batch *batch, int shot, float *out);
+/* Launch configuration chosen: cfg[0..3] = {elems/block, shots/group,
+ * graph block steps, groups}. */
+int sem_batch_config(const sem_batch *batch, int64_t *cfg);
+/* Timing of the dominant kernel (fused/stiffness) measured with HIP events
+ * on the batch stream during the last sem_batch_run: total ms and count. */
+int sem_batch_kernel_time(const sem_batch *batch, double *ms, int64_t *launches);
+
+/* Reference time-loop semantics (Options.cpp:84-97, Main.cpp:45): number
+ * of iterations of while (t < duration) and the adjusted dt. */
+int64_t sem_reference_step_count(double duration, double dt, double *dt_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SALVUS_SEM_H */

@@ -1,0 +1,821 @@
+// HIP kernels of the MI355X SEM path (gfx950).
+//
+// Element operator (per element, order N, n = N+1, nd = n^dim):
+//   g_a(q)   = sum_m D[q_a][m] u(q with q_a <- m)               reference gradient
+//   F_a(q)   = w_q detJ sum_c Jinv[c][a] sigma_c(q)              J^-T flux
+//   (K u)(p) = sum_a sum_m D[m][p_a] F_a(p with p_a <- m)        test-and-integrate
+// which is TensorQuad.cpp:176-210/:350-397 and Hexahedra.cpp:579-754 with
+// the reference's quadrature; for acoustics F = w vp^2 Gsym g with
+// Gsym = detJ Jinv^T Jinv (Scalar.cpp:42-71).  Stresses: Elastic2D.cpp:53-82
+// (C11,C13,C33,C55; C12 = C23 = 0), Elastic3D.cpp:33-54 / :70-118 (TI).
+//
+// One thread per element node; an element's field and fluxes are staged in
+// LDS; the 1-D derivative rows/columns live in registers.  Stiffness
+// arithmetic is FP32 with FP32 accumulation.  Geometry and mass are
+// computed in FP64 on the device and stored FP32.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "sem_internal.h"
+
+namespace sem {
+
+// ------------------------------------------------------------------ tables
+struct DevTables {
+  float D[kMaxN * kMaxN];  // D[i*n+m] = l'_m(xi_i)
+  float w[kMaxN];
+  float xi[kMaxN];
+  double wd[kMaxN];
+  double xid[kMaxN];
+};
+__constant__ DevTables c_tab[kMaxOrder2D + 1];
+
+template <int DIM, int N>
+struct Sh {
+  static constexpr int n = N + 1;
+  static constexpr int ND = DIM == 2 ? n * n : n * n * n;
+  static constexpr int NV = DIM == 2 ? 4 : 8;
+};
+
+__device__ __forceinline__ float p1_weight(int dim, int v, const float *xi3) {
+  // QuadP1.h:98-108 / HexP1.cpp:242-298 vertex signs
+  const int qs[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+  const int hs[8][3] = {{-1, -1, -1}, {-1, 1, -1}, {1, 1, -1}, {1, -1, -1},
+                        {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+  if (dim == 2) return 0.25f * (1.f + qs[v][0] * xi3[0]) * (1.f + qs[v][1] * xi3[1]);
+  return 0.125f * (1.f + hs[v][0] * xi3[0]) * (1.f + hs[v][1] * xi3[1]) * (1.f + hs[v][2] * xi3[2]);
+}
+
+// ------------------------------------------------------------------ FP64 geometry
+// Affine per-element factors from the Jacobian at the element centre.
+// Acoustic: Gsym = detJ Jinv^T Jinv (upper triangle, row-major: xx xy [xz] yy [yz] [zz]).
+// Elastic : Jinv (dim^2, [a][b] = d r_b / d x_a ... stored as Jinv[a*dim+b]) and detJ.
+__device__ void dev_jacobian(int dim, const double *vtx, const double *ref, double *J) {
+  const int qs[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+  const int hs[8][3] = {{-1, -1, -1}, {-1, 1, -1}, {1, 1, -1}, {1, -1, -1},
+                        {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+  for (int i = 0; i < dim * dim; ++i) J[i] = 0.0;
+  const int nv = dim == 2 ? 4 : 8;
+  for (int v = 0; v < nv; ++v) {
+    double d[3];
+    if (dim == 2) {
+      d[0] = 0.25 * qs[v][0] * (1 + ref[1] * qs[v][1]);
+      d[1] = 0.25 * qs[v][1] * (1 + ref[0] * qs[v][0]);
+    } else {
+      d[0] = 0.125 * hs[v][0] * (1 + ref[1] * hs[v][1]) * (1 + ref[2] * hs[v][2]);
+      d[1] = 0.125 * hs[v][1] * (1 + ref[0] * hs[v][0]) * (1 + ref[2] * hs[v][2]);
+      d[2] = 0.125 * hs[v][2] * (1 + ref[0] * hs[v][0]) * (1 + ref[1] * hs[v][1]);
+    }
+    for (int a = 0; a < dim; ++a)
+      for (int b = 0; b < dim; ++b) J[a * dim + b] += d[a] * vtx[v * dim + b];
+  }
+}
+
+__device__ double dev_invert(int dim, const double *J, double *inv) {
+  if (dim == 2) {
+    double d = J[0] * J[3] - J[1] * J[2];
+    inv[0] = J[3] / d; inv[1] = -J[1] / d; inv[2] = -J[2] / d; inv[3] = J[0] / d;
+    return d;
+  }
+  double c00 = J[4] * J[8] - J[5] * J[7], c01 = J[5] * J[6] - J[3] * J[8],
+         c02 = J[3] * J[7] - J[4] * J[6];
+  double d = J[0] * c00 + J[1] * c01 + J[2] * c02;
+  inv[0] = c00 / d; inv[1] = (J[2] * J[7] - J[1] * J[8]) / d; inv[2] = (J[1] * J[5] - J[2] * J[4]) / d;
+  inv[3] = c01 / d; inv[4] = (J[0] * J[8] - J[2] * J[6]) / d; inv[5] = (J[2] * J[3] - J[0] * J[5]) / d;
+  inv[6] = c02 / d; inv[7] = (J[1] * J[6] - J[0] * J[7]) / d; inv[8] = (J[0] * J[4] - J[1] * J[3]) / d;
+  return d;
+}
+
+// Interpolated vertex parameter q of one element at reference point ref (FP64).
+__device__ double dev_p1(int dim, const double *vpar, int npar, int q, const double *ref) {
+  const int qs[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+  const int hs[8][3] = {{-1, -1, -1}, {-1, 1, -1}, {1, 1, -1}, {1, -1, -1},
+                        {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+  double s = 0;
+  if (dim == 2) {
+    for (int v = 0; v < 4; ++v)
+      s += 0.25 * (1 + ref[0] * qs[v][0]) * (1 + ref[1] * qs[v][1]) * vpar[v * npar + q];
+  } else {
+    for (int v = 0; v < 8; ++v)
+      s += 0.125 * (1 + ref[0] * hs[v][0]) * (1 + ref[1] * hs[v][1]) * (1 + ref[2] * hs[v][2]) *
+           vpar[v * npar + q];
+  }
+  return s;
+}
+
+__global__ void k_affine_geometry(int dim, int physics, int64_t E, const double *__restrict__ vtx,
+                                  float *__restrict__ geo) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int nv = dim == 2 ? 4 : 8;
+  double J[9], inv[9], ref[3] = {0, 0, 0};
+  dev_jacobian(dim, vtx + e * nv * dim, ref, J);
+  double det = dev_invert(dim, J, inv);
+  if (physics == SEM_ACOUSTIC) {
+    const int ng = dim * (dim + 1) / 2;
+    int k = 0;
+    for (int a = 0; a < dim; ++a)
+      for (int b = a; b < dim; ++b) {
+        double s = 0;
+        for (int c = 0; c < dim; ++c) s += inv[c * dim + a] * inv[c * dim + b];
+        geo[e * ng + k++] = (float)(det * s);
+      }
+  } else {
+    const int ng = dim * dim + 1;
+    for (int i = 0; i < dim * dim; ++i) geo[e * ng + i] = (float)inv[i];
+    geo[e * ng + dim * dim] = (float)det;
+  }
+}
+
+// Pointwise factors (general geometry) and FP64 element mass, one thread
+// per (element, node).  geo_p layout: [E][ngp][nd].  me: [E][nd].
+template <int DIM, int N>
+__global__ void k_point_geometry(int physics, int64_t E, const double *__restrict__ vtx,
+                                 const double *__restrict__ vpar, int npar,
+                                 float *__restrict__ geo_p, double *__restrict__ me) {
+  using S = Sh<DIM, N>;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= E * S::ND) return;
+  int64_t e = t / S::ND;
+  int p = (int)(t % S::ND);
+  const DevTables &T = c_tab[N];
+  int P[3] = {p % S::n, (p / S::n) % S::n, DIM == 3 ? p / (S::n * S::n) : 0};
+  double ref[3] = {T.xid[P[0]], T.xid[P[1]], DIM == 3 ? T.xid[P[2]] : 0.0};
+  double wq = T.wd[P[0]] * T.wd[P[1]] * (DIM == 3 ? T.wd[P[2]] : 1.0);
+  double J[9], inv[9];
+  const double *ev = vtx + e * S::NV * DIM;
+  const double *ep = vpar + e * S::NV * npar;
+  dev_jacobian(DIM, ev, ref, J);
+  double det = dev_invert(DIM, J, inv);
+  // mass: Scalar.cpp:34-39 (no rho), Elastic*.cpp: rho at the node
+  double rho = physics == SEM_ACOUSTIC ? 1.0 : dev_p1(DIM, ep, npar, 0, ref);
+  if (me) me[e * S::ND + p] = wq * det * rho;
+  if (!geo_p) return;
+  const int ngp = point_geo_count(DIM, physics);
+  float *g = geo_p + e * ngp * S::ND + p;
+  if (physics == SEM_ACOUSTIC) {
+    double vp = dev_p1(DIM, ep, npar, 0, ref);
+    int k = 0;
+    for (int a = 0; a < DIM; ++a)
+      for (int b = a; b < DIM; ++b) {
+        double s = 0;
+        for (int c = 0; c < DIM; ++c) s += inv[c * DIM + a] * inv[c * DIM + b];
+        g[(k++) * S::ND] = (float)(wq * vp * vp * det * s);
+      }
+  } else {
+    for (int i = 0; i < DIM * DIM; ++i) g[i * S::ND] = (float)inv[i];
+    g[DIM * DIM * S::ND] = (float)(wq * det);
+    if (DIM == 2) {
+      for (int q = 0; q < 4; ++q) g[(5 + q) * S::ND] = (float)dev_p1(DIM, ep, npar, 1 + q, ref);
+    } else {
+      double r = dev_p1(DIM, ep, npar, 0, ref), vpv = dev_p1(DIM, ep, npar, 1, ref),
+             vph = dev_p1(DIM, ep, npar, 2, ref), vsv = dev_p1(DIM, ep, npar, 3, ref),
+             vsh = dev_p1(DIM, ep, npar, 4, ref), eta = dev_p1(DIM, ep, npar, 5, ref);
+      double c11 = r * vph * vph, c33 = r * vpv * vpv, c44 = r * vsv * vsv, c66 = r * vsh * vsh;
+      double cc[9] = {c11, c11 - 2 * c66, eta * (c11 - 2 * c44), c11, eta * (c11 - 2 * c44), c33,
+                      c44, c44, c66};  // c11 c12 c13 c22 c23 c33 c44 c55 c66
+      for (int q = 0; q < 9; ++q) g[(10 + q) * S::ND] = (float)cc[q];
+    }
+  }
+}
+
+// Colour-ordered FP64 mass assembly: one launch per colour.
+__global__ void k_mass_scatter(int64_t first, int64_t count, int nd, const uint32_t *__restrict__ conn,
+                               const double *__restrict__ me, double *__restrict__ M) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count * nd) return;
+  int64_t e = first + t / nd;
+  int p = (int)(t % nd);
+  uint32_t g = conn[e * nd + p] & kNodeMask;
+  M[g] += me[e * nd + p];
+}
+
+__global__ void k_reciprocal(int64_t G, const double *__restrict__ M, float *__restrict__ mi) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < G) mi[g] = (float)(1.0 / M[g]);
+}
+
+__global__ void k_fill_f64(int64_t n, double *p, double v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_f32(int64_t n, float *p, float v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_i32(int64_t n, int32_t *p, int32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Race check: count[g] += 1 for each element of one colour; nonzero >1 flags.
+__global__ void k_colour_touch(int64_t first, int64_t count, int nd, const uint32_t *__restrict__ conn,
+                               int32_t *__restrict__ hits) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count * nd) return;
+  uint32_t g = conn[(first + t / nd) * nd + t % nd] & kNodeMask;
+  atomicAdd(&hits[g], 1);
+}
+__global__ void k_colour_count(int64_t G, int32_t *__restrict__ hits, unsigned long long *viol) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  if (hits[g] > 1) atomicAdd(viol, (unsigned long long)(hits[g] - 1));
+  hits[g] = 0;
+}
+
+// ------------------------------------------------------------------ element operator
+// Per-thread cached 1-D operator rows/columns.
+template <int DIM, int N>
+struct NodeCtx {
+  using S = Sh<DIM, N>;
+  int p, P[3], base[3];
+  float Drow[DIM][N + 1], Dcol[DIM][N + 1];
+  float wq, xi[3];
+  __device__ void init(int p_) {
+    const DevTables &T = c_tab[N];
+    p = p_;
+    P[0] = p % S::n;
+    P[1] = (p / S::n) % S::n;
+    P[2] = DIM == 3 ? p / (S::n * S::n) : 0;
+    const int stride[3] = {1, S::n, S::n * S::n};
+    wq = 1.f;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      base[a] = p - P[a] * stride[a];
+      wq *= T.w[P[a]];
+      xi[a] = T.xi[P[a]];
+#pragma unroll
+      for (int m = 0; m < S::n; ++m) {
+        Drow[a][m] = T.D[P[a] * S::n + m];
+        Dcol[a][m] = T.D[m * S::n + P[a]];
+      }
+    }
+    if (DIM == 2) xi[2] = 0.f;
+  }
+  // reference gradient of the LDS field f (nd floats)
+  __device__ __forceinline__ void grad(const float *f, float *g) const {
+    const int stride[3] = {1, S::n, S::n * S::n};
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      float s = 0.f;
+#pragma unroll
+      for (int m = 0; m < S::n; ++m) s = fmaf(Drow[a][m], f[base[a] + m * stride[a]], s);
+      g[a] = s;
+    }
+  }
+  // test-and-integrate of LDS fluxes F[a*ND + q]
+  __device__ __forceinline__ float div(const float *F) const {
+    const int stride[3] = {1, S::n, S::n * S::n};
+    float s = 0.f;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+#pragma unroll
+      for (int m = 0; m < S::n; ++m) s = fmaf(Dcol[a][m], F[a * S::ND + base[a] + m * stride[a]], s);
+    return s;
+  }
+};
+
+// Material/geometry of one node, gathered once per element visit.
+template <int DIM, int PHYS>
+struct NodeMat {
+  // acoustic: G[6] (already times w vp^2);
+  // elastic : Jinv[DIM*DIM], wdet, c[9] (2D uses c[0..3] = C11,C13,C33,C55)
+  float G[6];
+  float Jinv[9];
+  float wdet;
+  float c[9];
+};
+
+template <int DIM, int N, int PHYS, bool AFFINE>
+__device__ __forceinline__ void load_mat(const NodeCtx<DIM, N> &X, int64_t e, const float *geo_e,
+                                         const float *par_e, const float *geo_p,
+                                         NodeMat<DIM, PHYS> &M) {
+  using S = Sh<DIM, N>;
+  if (AFFINE) {
+    float w8[8];
+#pragma unroll
+    for (int v = 0; v < S::NV; ++v) w8[v] = p1_weight(DIM, v, X.xi);
+    if (PHYS == SEM_ACOUSTIC) {
+      constexpr int ng = DIM * (DIM + 1) / 2;
+      const float *gp = geo_e + e * ng;
+      float vp = 0.f;
+#pragma unroll
+      for (int v = 0; v < S::NV; ++v) vp = fmaf(w8[v], par_e[e * S::NV + v], vp);
+      float s = X.wq * vp * vp;  // Scalar.cpp:45: vp^2 from P1-interpolated VP
+#pragma unroll
+      for (int k = 0; k < ng; ++k) M.G[k] = s * gp[k];
+    } else {
+      constexpr int ng = DIM * DIM + 1;
+      const float *gp = geo_e + e * ng;
+#pragma unroll
+      for (int k = 0; k < DIM * DIM; ++k) M.Jinv[k] = gp[k];
+      M.wdet = X.wq * gp[DIM * DIM];
+      if (DIM == 2) {
+        const float *pp = par_e + e * 4 * 4;  // 4 vertices x {C11,C13,C33,C55}
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          float s = 0.f;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) s = fmaf(w8[v], pp[v * 4 + q], s);
+          M.c[q] = s;
+        }
+      } else {
+        const float *pp = par_e + e * 8 * 6;  // 8 vertices x {RHO,VPV,VPH,VSV,VSH,ETA}
+        float q6[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          float s = 0.f;
+#pragma unroll
+          for (int v = 0; v < 8; ++v) s = fmaf(w8[v], pp[v * 6 + q], s);
+          q6[q] = s;
+        }
+        // Elastic3D.cpp:42-52
+        float c11 = q6[0] * q6[2] * q6[2], c33 = q6[0] * q6[1] * q6[1];
+        float c44 = q6[0] * q6[3] * q6[3], c66 = q6[0] * q6[4] * q6[4];
+        float c13 = q6[5] * (c11 - 2.f * c44);
+        M.c[0] = c11; M.c[1] = c11 - 2.f * c66; M.c[2] = c13; M.c[3] = c11; M.c[4] = c13;
+        M.c[5] = c33; M.c[6] = c44; M.c[7] = c44; M.c[8] = c66;
+      }
+    }
+  } else {
+    constexpr int ngp = PHYS == SEM_ACOUSTIC ? DIM * (DIM + 1) / 2 : (DIM == 2 ? 9 : 19);
+    const float *gp = geo_p + e * ngp * S::ND + X.p;
+    if (PHYS == SEM_ACOUSTIC) {
+#pragma unroll
+      for (int k = 0; k < ngp; ++k) M.G[k] = gp[k * S::ND];
+    } else {
+#pragma unroll
+      for (int k = 0; k < DIM * DIM; ++k) M.Jinv[k] = gp[k * S::ND];
+      M.wdet = gp[DIM * DIM * S::ND];
+#pragma unroll
+      for (int k = 0; k < (DIM == 2 ? 4 : 9); ++k) M.c[k] = gp[(DIM * DIM + 1 + k) * S::ND];
+    }
+  }
+}
+
+// Stage 1: fluxes of this node into LDS.  su: C x ND field, sF: C x DIM x ND.
+template <int DIM, int N, int PHYS>
+__device__ __forceinline__ void node_flux(const NodeCtx<DIM, N> &X, const NodeMat<DIM, PHYS> &M,
+                                          const float *su, float *sF) {
+  using S = Sh<DIM, N>;
+  if (PHYS == SEM_ACOUSTIC) {
+    float g[3];
+    X.grad(su, g);
+    if (DIM == 2) {
+      sF[0 * S::ND + X.p] = M.G[0] * g[0] + M.G[1] * g[1];
+      sF[1 * S::ND + X.p] = M.G[1] * g[0] + M.G[2] * g[1];
+    } else {
+      sF[0 * S::ND + X.p] = M.G[0] * g[0] + M.G[1] * g[1] + M.G[2] * g[2];
+      sF[1 * S::ND + X.p] = M.G[1] * g[0] + M.G[3] * g[1] + M.G[4] * g[2];
+      sF[2 * S::ND + X.p] = M.G[2] * g[0] + M.G[4] * g[1] + M.G[5] * g[2];
+    }
+  } else {
+    // physical gradients du_c/dx_a = sum_b Jinv[a][b] g_b
+    float du[3][3];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      float g[3];
+      X.grad(su + c * S::ND, g);
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) {
+        float s = 0.f;
+#pragma unroll
+        for (int b = 0; b < DIM; ++b) s = fmaf(M.Jinv[a * DIM + b], g[b], s);
+        du[c][a] = s;
+      }
+    }
+    float sig[3][3];
+    if (DIM == 2) {
+      // Elastic2D.cpp:53-82 (mc13 = mc23 = 0)
+      float exx = du[0][0], eyy = du[1][1], sh = du[0][1] + du[1][0];
+      sig[0][0] = M.c[0] * exx + M.c[1] * eyy;
+      sig[1][1] = M.c[1] * exx + M.c[2] * eyy;
+      sig[0][1] = sig[1][0] = M.c[3] * sh;
+    } else {
+      // Elastic3D.cpp:76-83, :106-118
+      float e0 = du[0][0], e1 = du[1][1], e2 = du[2][2];
+      float e3 = du[1][2] + du[2][1], e4 = du[0][2] + du[2][0], e5 = du[0][1] + du[1][0];
+      sig[0][0] = M.c[0] * e0 + M.c[1] * e1 + M.c[2] * e2;
+      sig[1][1] = M.c[1] * e0 + M.c[3] * e1 + M.c[4] * e2;
+      sig[2][2] = M.c[2] * e0 + M.c[4] * e1 + M.c[5] * e2;
+      sig[1][2] = sig[2][1] = M.c[6] * e3;
+      sig[0][2] = sig[2][0] = M.c[7] * e4;
+      sig[0][1] = sig[1][0] = M.c[8] * e5;
+    }
+    // F_{c,a} = w detJ sum_b Jinv[b][a] sigma_{c,b}
+#pragma unroll
+    for (int c = 0; c < DIM; ++c)
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) {
+        float s = 0.f;
+#pragma unroll
+        for (int b = 0; b < DIM; ++b) s = fmaf(M.Jinv[b * DIM + a], sig[c][b], s);
+        sF[(c * DIM + a) * S::ND + X.p] = M.wdet * s;
+      }
+  }
+}
+
+// ------------------------------------------------------------------ fused colour kernel
+// One launch per colour.  Each block takes one work item (EPB elements of
+// one group); for every shot of the group it gathers u, applies the element
+// operator, and for every node either stores (first colour), accumulates,
+// or -- on the node's last colour -- finishes the step there:
+//   a = mi (acc + f - K u); Kahan v += dt/2 (a + a_); Kahan u += dt v + dt^2/2 a; a_ = a
+// (Order2Newmark.cpp:68-115).  Sources add their nodal load times R(t_n).
+
+
+__device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
+  float y = x - c;
+  float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+template <int DIM, int N, int PHYS, bool AFFINE>
+__global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
+  using S = Sh<DIM, N>;
+  constexpr int C = PHYS == SEM_ACOUSTIC ? 1 : DIM;
+  extern __shared__ float lds[];
+  // XCD-aware item order: consecutive hardware block ids round-robin over
+  // 8 XCDs; give each XCD a contiguous range of items.
+  int bid = blockIdx.x, nb = gridDim.x;
+  int item = bid;
+  if (nb >= 16 && (nb & 7) == 0) item = (bid & 7) * (nb >> 3) + (bid >> 3);
+  if (item >= A.nitems) return;
+  const WorkItem W = A.items[item];
+  const GroupDev &G = A.groups[W.group];
+  const int64_t step = *G.step + A.koff;
+  if (step >= G.nsteps) return;  // uniform across the block
+
+  const int t = threadIdx.x;
+  const int el = t / S::ND;
+  const int p = t - el * S::ND;
+  const bool active = el < W.count;
+  float *su = lds + el * (C * S::ND);
+  float *sF = lds + A.epb * C * S::ND + el * (C * DIM * S::ND);
+
+  NodeCtx<DIM, N> X;
+  NodeMat<DIM, PHYS> M;
+  int64_t e = W.first + el;
+  uint32_t word = 0, g = 0;
+  float mi = 0.f;
+  if (active) {
+    X.init(p);
+    word = G.conn[e * S::ND + p];
+    g = word & kNodeMask;
+    load_mat<DIM, N, PHYS, AFFINE>(X, e, G.geo_e, G.par_e, G.geo_p, M);
+    if (word & kFlagLast) mi = G.mi[g];
+  }
+  const int Sg = G.S;
+  for (int s = 0; s < Sg; ++s) {
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) su[c * S::ND + p] = G.u[((int64_t)c * G.G + g) * Sg + s];
+    }
+    __syncthreads();
+    if (active) node_flux<DIM, N, PHYS>(X, M, su, sF);
+    __syncthreads();
+    if (active) {
+      float f[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        float ku = X.div(sF + c * DIM * S::ND);
+        f[c] = (word & kFlagDirichlet) ? 0.f : -ku;
+      }
+      for (int k = 0; k < G.nsrc; ++k) {
+        const SrcDev sd = G.src[k];
+        if (sd.elem == e && sd.shot == s) {
+          float R = G.stf[sd.stf_off + step];
+#pragma unroll
+          for (int c = 0; c < C; ++c) f[c] = fmaf(G.load[sd.load_off + c * S::ND + p], R, f[c]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int64_t idx = ((int64_t)c * G.G + g) * Sg + s;
+        if (!(word & kFlagLast) || A.ku_only) {
+          if (word & kFlagFirst) G.acc[idx] = f[c];
+          else G.acc[idx] += f[c];
+        } else {
+          f[c] = (word & kFlagFirst) ? f[c] : G.acc[idx] + f[c];
+        }
+      }
+      if ((word & kFlagLast) && !A.ku_only) {
+        // Order2Newmark.cpp:68-115; with the 3-D quirk the list is {x, y, x}
+        const bool q3 = (C == 3) && (G.quirks & SEM_QUIRK_NEWMARK3D);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          if (q3 && c == 2) continue;
+          const int64_t idx = ((int64_t)c * G.G + g) * Sg + s;
+          float a = mi * f[c];
+          float ap = G.ap[idx];
+          float v = G.v[idx], kv = G.kv[idx];
+          float u = su[c * S::ND + p], ku = G.ku[idx];
+          const int reps = (q3 && c == 0) ? 2 : 1;
+          for (int r = 0; r < reps; ++r) {
+            kahan_add(v, kv, G.acl * (a + ap));
+            kahan_add(u, ku, fmaf(G.dt, v, G.dsp * a));
+            ap = a;
+          }
+          G.u[idx] = u; G.v[idx] = v; G.ap[idx] = a; G.ku[idx] = ku; G.kv[idx] = kv;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ plain path
+// One shot per launch; S = 1 layout.  gather -> stiffness -> colour scatter
+// -> update, each a separate kernel with element-local buffers in HBM.
+__global__ void k_plain_gather(int64_t E, int nd, int C, int64_t G, const uint32_t *__restrict__ conn,
+                               const float *__restrict__ u, float *__restrict__ uloc) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= E * nd) return;
+  uint32_t g = conn[t] & kNodeMask;
+  int64_t e = t / nd;
+  int p = (int)(t % nd);
+  for (int c = 0; c < C; ++c) uloc[(e * C + c) * nd + p] = u[(int64_t)c * G + g];
+}
+
+
+
+template <int DIM, int N, int PHYS>
+__global__ void __launch_bounds__(512) k_plain_stiffness(PlainArgs A) {
+  using S = Sh<DIM, N>;
+  constexpr int C = PHYS == SEM_ACOUSTIC ? 1 : DIM;
+  extern __shared__ float lds[];
+  const int t = threadIdx.x;
+  const int el = t / S::ND;
+  const int p = t - el * S::ND;
+  const int64_t e = (int64_t)blockIdx.x * A.epb + el;
+  const bool active = el < A.epb && e < A.E;
+  float *su = lds + el * (C * S::ND);
+  float *sF = lds + A.epb * C * S::ND + el * (C * DIM * S::ND);
+  NodeCtx<DIM, N> X;
+  NodeMat<DIM, PHYS> M;
+  if (active) {
+    X.init(p);
+    load_mat<DIM, N, PHYS, false>(X, e, nullptr, nullptr, A.geo_p, M);
+#pragma unroll
+    for (int c = 0; c < C; ++c) su[c * S::ND + p] = A.uloc[(e * C + c) * S::ND + p];
+  }
+  __syncthreads();
+  if (active) node_flux<DIM, N, PHYS>(X, M, su, sF);
+  __syncthreads();
+  if (!active) return;
+  const uint32_t word = A.conn[e * S::ND + p];
+  const int64_t step = A.step ? *A.step + A.koff : 0;
+  float f[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    float ku = X.div(sF + c * DIM * S::ND);
+    f[c] = (word & kFlagDirichlet) ? 0.f : -ku;
+  }
+  for (int k = 0; k < A.nsrc; ++k) {
+    if (A.src[k].elem == e) {
+      float R = A.stf[A.src[k].stf_off + step];
+#pragma unroll
+      for (int c = 0; c < C; ++c) f[c] = fmaf(A.load[A.src[k].load_off + c * S::ND + p], R, f[c]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < C; ++c) A.aloc[(e * C + c) * S::ND + p] = f[c];
+}
+
+__global__ void k_plain_scatter(int64_t first, int64_t count, int nd, int C, int64_t G,
+                                const uint32_t *__restrict__ conn, const float *__restrict__ aloc,
+                                float *__restrict__ acc) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count * nd) return;
+  int64_t e = first + t / nd;
+  int p = (int)(t % nd);
+  uint32_t g = conn[e * nd + p] & kNodeMask;
+  for (int c = 0; c < C; ++c) acc[(int64_t)c * G + g] += aloc[(e * C + c) * nd + p];
+}
+
+__global__ void k_plain_update(int64_t G, int C, int quirks, float dt, float acl, float dsp,
+                               const float *__restrict__ mi, const float *__restrict__ acc,
+                               float *__restrict__ u, float *__restrict__ v, float *__restrict__ ap,
+                               float *__restrict__ ku, float *__restrict__ kv) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const bool q3 = C == 3 && (quirks & SEM_QUIRK_NEWMARK3D);
+  float m = mi[g];
+  for (int c = 0; c < C; ++c) {
+    if (q3 && c == 2) continue;
+    int64_t i = (int64_t)c * G + g;
+    float a = m * acc[i];
+    float pa = ap[i], vv = v[i], kvv = kv[i], uu = u[i], kuu = ku[i];
+    int reps = (q3 && c == 0) ? 2 : 1;
+    for (int r = 0; r < reps; ++r) {
+      kahan_add(vv, kvv, acl * (a + pa));
+      kahan_add(uu, kuu, fmaf(dt, vv, dsp * a));
+      pa = a;
+    }
+    u[i] = uu; v[i] = vv; ap[i] = a; ku[i] = kuu; kv[i] = kvv;
+  }
+}
+
+// ------------------------------------------------------------------ receivers / steps
+
+
+// Samples u after the step just taken (global step = *step + koff).
+__global__ void k_receivers(RecArgs A, int C) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= A.nrec * C) return;
+  int r = t / C, c = t % C;
+  const GroupDev &G = A.groups[A.rec_group[r]];
+  const int64_t step = *G.step + A.koff;
+  if (step >= G.nsteps) return;
+  const RecDev R = A.recs[r];
+  const uint32_t *cn = G.conn + (int64_t)R.elem * A.nd;
+  float s = 0.f;
+  for (int p = 0; p < A.nd; ++p)
+    s = fmaf(A.weights[R.w_off + p], G.u[((int64_t)c * G.G + (cn[p] & kNodeMask)) * G.S + R.shot], s);
+  A.out[R.out_off + step * R.stride + c] = s;
+}
+
+__global__ void k_advance(int ngroups, const GroupDev *groups, int64_t *steps, int64_t k) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ngroups) return;
+  int64_t s = steps[i] + k;
+  steps[i] = s < groups[i].nsteps ? s : groups[i].nsteps;
+}
+
+}  // namespace sem
+
+// ====================================================================== launchers
+namespace sem {
+
+namespace {
+inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+int last_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return -2;
+  }
+  return 0;
+}
+}  // namespace
+
+int init_device_tables(int device) {
+  static bool done[64] = {false};
+  if (device < 0 || device >= 64) { set_error("device index"); return -1; }
+  if (done[device]) return 0;
+  DevTables host[kMaxOrder2D + 1];
+  memset(host, 0, sizeof host);
+  for (int N = 1; N <= kMaxOrder2D; ++N) {
+    Gll g = make_gll(N);
+    for (int i = 0; i < g.n * g.n; ++i) host[N].D[i] = (float)g.D[i];
+    for (int i = 0; i < g.n; ++i) {
+      host[N].w[i] = (float)g.w[i];
+      host[N].xi[i] = (float)g.xi[i];
+      host[N].wd[i] = g.w[i];
+      host[N].xid[i] = g.xi[i];
+    }
+  }
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab), host, sizeof host, 0, hipMemcpyHostToDevice);
+  if (e != hipSuccess) { set_error(std::string("GLL tables: ") + hipGetErrorString(e)); return -2; }
+  done[device] = true;
+  return 0;
+}
+
+int launch_affine_geometry(int dim, int physics, int64_t E, const double *vtx, float *geo,
+                           hipStream_t st) {
+  k_affine_geometry<<<nblocks(E, 128), 128, 0, st>>>(dim, physics, E, vtx, geo);
+  return last_launch("affine geometry");
+}
+
+#define SEM_ORDERS_3D(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+#define SEM_ORDERS_2D(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
+
+int launch_point_geometry(int dim, int order, int physics, int64_t E, const double *vtx,
+                          const double *vpar, int npar, float *geo_p, double *me, hipStream_t st) {
+  const int n = order + 1;
+  const int64_t total = E * (dim == 2 ? n * n : n * n * n);
+#define PG3(N) case N: k_point_geometry<3, N><<<nblocks(total, 128), 128, 0, st>>>(physics, E, vtx, vpar, npar, geo_p, me); break;
+#define PG2(N) case N: k_point_geometry<2, N><<<nblocks(total, 128), 128, 0, st>>>(physics, E, vtx, vpar, npar, geo_p, me); break;
+  if (dim == 3) {
+    switch (order) { SEM_ORDERS_3D(PG3) default: set_error("order not compiled"); return -1; }
+  } else {
+    switch (order) { SEM_ORDERS_2D(PG2) default: set_error("order not compiled"); return -1; }
+  }
+  return last_launch("point geometry");
+}
+
+int launch_mass_scatter(int64_t first, int64_t count, int nd, const uint32_t *conn,
+                        const double *me, double *M, hipStream_t st) {
+  if (count <= 0) return 0;
+  k_mass_scatter<<<nblocks(count * nd, 256), 256, 0, st>>>(first, count, nd, conn, me, M);
+  return last_launch("mass scatter");
+}
+int launch_reciprocal(int64_t G, const double *M, float *mi, hipStream_t st) {
+  k_reciprocal<<<nblocks(G, 256), 256, 0, st>>>(G, M, mi);
+  return last_launch("reciprocal");
+}
+int launch_fill_f64(int64_t n, double *p, double v, hipStream_t st) {
+  if (n <= 0) return 0;
+  k_fill_f64<<<nblocks(n, 256), 256, 0, st>>>(n, p, v);
+  return last_launch("fill");
+}
+int launch_fill_f32(int64_t n, float *p, float v, hipStream_t st) {
+  if (n <= 0) return 0;
+  k_fill_f32<<<nblocks(n, 256), 256, 0, st>>>(n, p, v);
+  return last_launch("fill");
+}
+int launch_fill_i32(int64_t n, int32_t *p, int32_t v, hipStream_t st) {
+  if (n <= 0) return 0;
+  k_fill_i32<<<nblocks(n, 256), 256, 0, st>>>(n, p, v);
+  return last_launch("fill");
+}
+int launch_colour_touch(int64_t first, int64_t count, int nd, const uint32_t *conn, int32_t *hits,
+                        hipStream_t st) {
+  if (count <= 0) return 0;
+  k_colour_touch<<<nblocks(count * nd, 256), 256, 0, st>>>(first, count, nd, conn, hits);
+  return last_launch("colour touch");
+}
+int launch_colour_count(int64_t G, int32_t *hits, unsigned long long *viol, hipStream_t st) {
+  k_colour_count<<<nblocks(G, 256), 256, 0, st>>>(G, hits, viol);
+  return last_launch("colour count");
+}
+
+int launch_fused(int dim, int order, int physics, bool affine, int nitems, int epb, int nd, int c,
+                 const FusedArgs &a, hipStream_t st) {
+  if (nitems <= 0) return 0;
+  const int bt = block_threads(nd, epb);
+  if (bt > 512) { set_error("elements per block too large"); return -1; }
+  const size_t shm = fused_lds_bytes(dim, nd, c, epb);
+  dim3 grid(nitems), block(bt);
+#define FU(D, N, P, AF) hipLaunchKernelGGL((k_fused<D, N, P, AF>), grid, block, shm, st, a)
+#define FU4(D, N)                                      \
+  case N:                                              \
+    if (physics == SEM_ACOUSTIC) {                     \
+      if (affine) FU(D, N, 0, true); else FU(D, N, 0, false); \
+    } else {                                           \
+      if (affine) FU(D, N, 1, true); else FU(D, N, 1, false); \
+    }                                                  \
+    break;
+#define FU3(N) FU4(3, N)
+#define FU2(N) FU4(2, N)
+  if (dim == 3) {
+    switch (order) { SEM_ORDERS_3D(FU3) default: set_error("order not compiled"); return -1; }
+  } else {
+    switch (order) { SEM_ORDERS_2D(FU2) default: set_error("order not compiled"); return -1; }
+  }
+  return last_launch("fused step");
+}
+
+int launch_plain_gather(int64_t E, int nd, int C, int64_t G, const uint32_t *conn, const float *u,
+                        float *uloc, hipStream_t st) {
+  k_plain_gather<<<nblocks(E * nd, 256), 256, 0, st>>>(E, nd, C, G, conn, u, uloc);
+  return last_launch("plain gather");
+}
+
+int launch_plain_stiffness(int dim, int order, int physics, int64_t E, int nd, int c,
+                           const PlainArgs &a, hipStream_t st) {
+  const int bt = block_threads(nd, a.epb);
+  const size_t shm = fused_lds_bytes(dim, nd, c, a.epb);
+  dim3 grid(nblocks(E, a.epb)), block(bt);
+#define PS(D, N) case N: \
+    if (physics == SEM_ACOUSTIC) hipLaunchKernelGGL((k_plain_stiffness<D, N, 0>), grid, block, shm, st, a); \
+    else hipLaunchKernelGGL((k_plain_stiffness<D, N, 1>), grid, block, shm, st, a); \
+    break;
+#define PS3(N) PS(3, N)
+#define PS2(N) PS(2, N)
+  if (dim == 3) {
+    switch (order) { SEM_ORDERS_3D(PS3) default: set_error("order not compiled"); return -1; }
+  } else {
+    switch (order) { SEM_ORDERS_2D(PS2) default: set_error("order not compiled"); return -1; }
+  }
+  return last_launch("plain stiffness");
+}
+
+int launch_plain_scatter(int64_t first, int64_t count, int nd, int C, int64_t G,
+                         const uint32_t *conn, const float *aloc, float *acc, hipStream_t st) {
+  if (count <= 0) return 0;
+  k_plain_scatter<<<nblocks(count * nd, 256), 256, 0, st>>>(first, count, nd, C, G, conn, aloc, acc);
+  return last_launch("plain scatter");
+}
+
+int launch_plain_update(int64_t G, int C, int quirks, float dt, float acl, float dsp,
+                        const float *mi, const float *acc, float *u, float *v, float *ap,
+                        float *ku, float *kv, hipStream_t st) {
+  k_plain_update<<<nblocks(G, 256), 256, 0, st>>>(G, C, quirks, dt, acl, dsp, mi, acc, u, v, ap, ku, kv);
+  return last_launch("plain update");
+}
+
+int launch_receivers(const RecArgs &a, int C, hipStream_t st) {
+  if (a.nrec <= 0) return 0;
+  k_receivers<<<nblocks((int64_t)a.nrec * C, 128), 128, 0, st>>>(a, C);
+  return last_launch("receivers");
+}
+
+int launch_advance(int ngroups, const GroupDev *groups, int64_t *steps, int64_t k, hipStream_t st) {
+  k_advance<<<nblocks(ngroups, 64), 64, 0, st>>>(ngroups, groups, steps, k);
+  return last_launch("advance");
+}
+
+}  // namespace sem

@@ -1,0 +1,238 @@
+// Internal declarations of the MI355X SEM library (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/salvus_sem.h"
+
+namespace sem {
+
+// ---------------------------------------------------------------- limits
+constexpr int kMaxOrder3D = 7;   // Hexahedra::mMaxOrder with the order-7 table
+constexpr int kMaxOrder2D = 10;  // TensorQuad::mMaxOrder
+constexpr int kMaxN = 11;
+
+// Gather-map words carry three flags above the node index (G < 2^29).
+constexpr uint32_t kNodeMask = (1u << 29) - 1;
+constexpr uint32_t kFlagDirichlet = 1u << 29;  // zero the stiffness row (Q14)
+constexpr uint32_t kFlagFirst = 1u << 30;      // first colour to touch the node: store
+constexpr uint32_t kFlagLast = 1u << 31;       // last colour: finalise (mass + Newmark)
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+#define SEM_TRY(expr)                                   \
+  do {                                                  \
+    int _rc = (expr);                                   \
+    if (_rc) return _rc;                                \
+  } while (0)
+int hip_check(hipError_t e, const char *what);
+#define HIPCHK(x) SEM_TRY(::sem::hip_check((x), #x))
+
+// ---------------------------------------------------------------- host maths (FP64)
+struct Gll {
+  int order = 0, n = 0;
+  double xi[kMaxN], w[kMaxN], D[kMaxN * kMaxN];  // D[i*n+j] = l'_j(xi_i)
+};
+Gll make_gll(int order);
+void lagrange(const Gll &g, double x, double *l);
+void lagrange_deriv(const Gll &g, double x, double *dl);
+
+// J rows d/dr, d/ds(, d/dt); returns det.
+double jacobian(int dim, const double *vtx, const double *ref, double *J);
+double invert(int dim, const double *J, double *inv);
+void p1_weights(int dim, const double *ref, double *w);
+// 1 inside/on hull, 0 outside, -1 degenerate element
+int check_hull(int dim, const double *pt, const double *vtx);
+int inverse_map(int dim, const double *pt, const double *vtx, double *ref);
+
+// ---------------------------------------------------------------- mesh topology
+struct Topology {
+  int dim = 0, order = 0;
+  int64_t E = 0, V = 0, NF = 0, NEd = 0, G = 0;
+  std::vector<int32_t> conn;  // E x nd (original element order)
+};
+int build_topology(int dim, int order, int64_t E, int64_t V, const int32_t *cells, Topology &T);
+int colour_elements(int dim, int64_t E, int64_t V, const int32_t *cells,
+                    std::vector<int32_t> &colour);
+
+// ---------------------------------------------------------------- device-side descriptors
+// Per-element factor counts.
+// Affine geometry (one Jacobian per element): acoustic Gsym (dim*(dim+1)/2);
+// elastic Jinv (dim^2) + detJ (1).
+// Pointwise geometry (per GLL node): acoustic w*vp^2*Gsym; elastic 2D
+// Jinv(4)+w detJ(1)+C11,C13,C33,C55(4); elastic 3D Jinv(9)+w detJ(1)+9 c's.
+__host__ __device__ inline int affine_geo_count(int dim, int physics) {
+  return physics == SEM_ACOUSTIC ? dim * (dim + 1) / 2 : dim * dim + 1;
+}
+__host__ __device__ inline int point_geo_count(int dim, int physics) {
+  if (physics == SEM_ACOUSTIC) return dim * (dim + 1) / 2;
+  return dim == 2 ? 9 : 19;
+}
+// Vertex material the fused kernels interpolate on the fly (stiffness only):
+// acoustic {VP}; Elastic2D {C11,C13,C33,C55}; Elastic3D {RHO,VPV,VPH,VSV,VSH,ETA}.
+__host__ __device__ inline int kernel_npar(int dim, int physics) {
+  if (physics == SEM_ACOUSTIC) return 1;
+  return dim == 2 ? 4 : 6;
+}
+__host__ __device__ inline int ref_npar(int dim, int physics) {
+  if (physics == SEM_ACOUSTIC) return 1;
+  return dim == 2 ? 5 : 6;
+}
+
+struct SrcDev {
+  int32_t elem;     // sorted element position
+  int32_t stf_off;  // offset of this source's STF samples
+  int32_t load_off; // offset of nd*c load weights
+  int32_t shot;     // shot slot within the group
+};
+
+struct RecDev {
+  int32_t elem;      // sorted element position
+  int32_t w_off;     // offset of nd Lagrange weights
+  int32_t shot;      // shot slot within the group
+  int32_t stride;    // floats between consecutive steps (R_shot * c)
+  int64_t out_off;   // offset of this receiver's step-0, component-0 sample
+};
+
+// One group = shots sharing a mesh, fields interleaved shot-fastest:
+// field[(comp*G + g)*S + s].
+struct GroupDev {
+  const uint32_t *conn;   // E x nd (sorted order, flags)
+  const float *geo_e;     // affine: E x ng
+  const float *par_e;     // E x nv x kpar
+  const float *geo_p;     // pointwise: E x ngp x nd
+  const float *mi;        // G
+  float *u, *v, *ap, *ku, *kv, *acc;
+  const SrcDev *src;
+  const float *stf;
+  const float *load;
+  int32_t nsrc;
+  int32_t S;
+  int64_t G;
+  int64_t nsteps;
+  const int64_t *step;    // device step counter of this group
+  float dt, acl, dsp;
+  int32_t quirks;
+  int32_t pad;
+};
+
+// Work item of a colour launch: EPB elements of one group.
+struct WorkItem {
+  int32_t group;
+  int32_t first;  // first sorted element position
+  int32_t count;  // elements in this item
+  int32_t pad;
+};
+
+// Kernel argument blocks (passed by value; recorded by graph capture).
+struct FusedArgs {
+  const GroupDev *groups;
+  const WorkItem *items;
+  int32_t nitems;
+  int32_t epb;
+  int32_t koff;      // step offset within the captured block
+  int32_t ku_only;   // 1: accumulate -K u into acc on every colour (no update)
+};
+
+struct PlainArgs {
+  const uint32_t *conn;
+  const float *geo_p;
+  const float *uloc;
+  float *aloc;
+  const SrcDev *src;
+  const float *stf;
+  const float *load;
+  const int64_t *step;
+  int32_t nsrc;
+  int32_t epb;
+  int64_t E;
+  int32_t koff;
+  int32_t pad;
+};
+
+struct RecArgs {
+  const GroupDev *groups;
+  const int32_t *rec_group;  // group of each receiver
+  const RecDev *recs;
+  const float *weights;
+  float *out;
+  int32_t nrec;
+  int32_t nd;
+  int32_t koff;
+  int32_t pad;
+};
+
+// ---------------------------------------------------------------- launchers (kernels.hip)
+int init_device_tables(int device);
+int launch_affine_geometry(int dim, int physics, int64_t E, const double *vtx, float *geo,
+                           hipStream_t st);
+int launch_point_geometry(int dim, int order, int physics, int64_t E, const double *vtx,
+                          const double *vpar, int npar, float *geo_p, double *me, hipStream_t st);
+int launch_mass_scatter(int64_t first, int64_t count, int nd, const uint32_t *conn,
+                        const double *me, double *M, hipStream_t st);
+int launch_reciprocal(int64_t G, const double *M, float *mi, hipStream_t st);
+int launch_fill_f64(int64_t n, double *p, double v, hipStream_t st);
+int launch_fill_f32(int64_t n, float *p, float v, hipStream_t st);
+int launch_fill_i32(int64_t n, int32_t *p, int32_t v, hipStream_t st);
+int launch_colour_touch(int64_t first, int64_t count, int nd, const uint32_t *conn, int32_t *hits,
+                        hipStream_t st);
+int launch_colour_count(int64_t G, int32_t *hits, unsigned long long *viol, hipStream_t st);
+// Threads per block for EPB elements of nd nodes (multiple of 64).
+inline int block_threads(int nd, int epb) { return ((nd * epb + 63) / 64) * 64; }
+inline size_t fused_lds_bytes(int dim, int nd, int c, int epb) {
+  return sizeof(float) * (size_t)epb * c * nd * (1 + dim);
+}
+int launch_fused(int dim, int order, int physics, bool affine, int nitems, int epb, int nd, int c,
+                 const FusedArgs &a, hipStream_t st);
+int launch_plain_gather(int64_t E, int nd, int C, int64_t G, const uint32_t *conn, const float *u,
+                        float *uloc, hipStream_t st);
+int launch_plain_stiffness(int dim, int order, int physics, int64_t E, int nd, int c,
+                           const PlainArgs &a, hipStream_t st);
+int launch_plain_scatter(int64_t first, int64_t count, int nd, int C, int64_t G,
+                         const uint32_t *conn, const float *aloc, float *acc, hipStream_t st);
+int launch_plain_update(int64_t G, int C, int quirks, float dt, float acl, float dsp,
+                        const float *mi, const float *acc, float *u, float *v, float *ap,
+                        float *ku, float *kv, hipStream_t st);
+int launch_receivers(const RecArgs &a, int C, hipStream_t st);
+int launch_advance(int ngroups, const GroupDev *groups, int64_t *steps, int64_t k, hipStream_t st);
+
+}  // namespace sem
+
+// ---------------------------------------------------------------- public structs
+struct sem_mesh {
+  int device = 0, dim = 0, order = 0, n = 0, nd = 0, physics = 0, c = 0, nv = 0, quirks = 0;
+  int64_t E = 0, V = 0, G = 0;
+  int ncol = 0;
+  bool affine = false;
+  std::vector<int64_t> col_start;  // ncol+1 positions in sorted order
+  std::vector<int32_t> perm;       // sorted position -> original element
+  std::vector<int32_t> inv_perm;   // original element -> sorted position
+  std::vector<int32_t> conn;       // original order E x nd (plain)
+  std::vector<int32_t> colour;     // original order
+  std::vector<double> vtx;         // original order E x nv x dim
+  std::vector<double> vpar;        // original order E x nv x npar
+  std::vector<double> bbox;        // original order E x 2 dim
+  std::vector<uint8_t> dirichlet;  // G (empty = none)
+  // device
+  hipStream_t stream = nullptr;
+  uint32_t *d_conn = nullptr;      // sorted E x nd with flags
+  float *d_geo_e = nullptr;        // affine per-element factors
+  float *d_par_e = nullptr;        // per-element vertex material (kernel set)
+  float *d_geo_p = nullptr;        // pointwise factors (lazy)
+  float *d_mi = nullptr;
+  double *d_vtx = nullptr;         // sorted E x nv x dim (FP64)
+  double *d_vpar = nullptr;        // sorted E x nv x npar (FP64)
+  size_t bytes = 0;
+};
+
+namespace sem {
+struct Shot;
+struct Group;
+}  // namespace sem
+
+struct sem_batch;

@@ -1,0 +1,338 @@
+"""Python front end of the MI355X SEM library (salvus_amd/_lib/libsalvus_sem.so).
+
+Thin ctypes bindings over the C ABI declared in include/salvus_sem.h, with
+classes named after the reference's plugin/operator API:
+
+  Mesh            Mesh::read(dim, ...) + setupGlobalDof + element set-up
+                  (Mesh.cpp:31-51, :361-452; Order2Newmark.cpp:32-66)
+  Batch           shots advanced together through the fused or plain path
+  Order2Newmark   one shot driven like the reference's Problem loop
+                  (Problem.cpp:143-215, Order2Newmark.cpp:68-115, Main.cpp:45-66)
+
+The HIP library must be built (python -c "import __graft_entry__ as g;
+g.build()"); nothing here falls back to a CPU path: a missing library or a
+missing GPU raises.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libsalvus_sem.so"
+
+ACOUSTIC, ELASTIC = 0, 1
+QUIRK_NEWMARK3D, QUIRK_E3D_SOURCE = 1, 2
+DEFAULT_QUIRKS = QUIRK_E3D_SOURCE
+PATH_PLAIN, PATH_FUSED = 0, 1
+FIELD = {"u": 0, "v": 1, "a_": 2}
+
+# Every entry point of include/salvus_sem.h.
+EXPORTS = (
+    "sem_last_error", "sem_device_count", "sem_topology_build", "sem_mesh_create",
+    "sem_mesh_destroy", "sem_mesh_info", "sem_mesh_get_conn", "sem_mesh_get_colours",
+    "sem_mesh_get_inverse_mass", "sem_mesh_get_node_coords", "sem_mesh_set_dirichlet",
+    "sem_mesh_apply_stiffness", "sem_mesh_check_colouring", "sem_batch_create",
+    "sem_batch_destroy", "sem_batch_add_shot", "sem_shot_add_source",
+    "sem_shot_add_moment_source", "sem_shot_add_receiver", "sem_shot_set_field",
+    "sem_shot_get_field", "sem_batch_prepare", "sem_batch_run", "sem_shot_progress",
+    "sem_shot_get_seismograms", "sem_batch_config", "sem_batch_kernel_time",
+    "sem_reference_step_count",
+)
+
+_lib = None
+_D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
+_I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
+_VP = C.c_void_p
+
+
+class SemError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise SemError("HIP library %s is not built; run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(str(LIB_PATH))
+        L.sem_last_error.restype = C.c_char_p
+        L.sem_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.sem_topology_build.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_int64, _I32, _I32, _I32,
+                                         _I64]
+        L.sem_mesh_create.argtypes = [C.POINTER(_VP), C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.c_int64, C.c_int64, _I32, _D, _D, C.c_int]
+        L.sem_mesh_destroy.argtypes = [_VP]
+        L.sem_mesh_info.argtypes = [_VP, _I64]
+        L.sem_mesh_get_conn.argtypes = [_VP, _I32]
+        L.sem_mesh_get_colours.argtypes = [_VP, _I32]
+        L.sem_mesh_get_inverse_mass.argtypes = [_VP, _F]
+        L.sem_mesh_get_node_coords.argtypes = [_VP, _D]
+        L.sem_mesh_set_dirichlet.argtypes = [_VP, C.POINTER(C.c_uint8)]
+        L.sem_mesh_apply_stiffness.argtypes = [_VP, C.c_int, _F, _F]
+        L.sem_mesh_check_colouring.argtypes = [_VP, _I64]
+        L.sem_batch_create.argtypes = [C.POINTER(_VP), C.c_int]
+        L.sem_batch_destroy.argtypes = [_VP]
+        L.sem_batch_add_shot.argtypes = [_VP, _VP, C.c_int64, C.c_double, C.POINTER(C.c_int)]
+        L.sem_shot_add_source.argtypes = [_VP, C.c_int, _D, C.c_double, C.c_double, C.c_double,
+                                          _D]
+        L.sem_shot_add_moment_source.argtypes = [_VP, C.c_int, _D, _D, C.c_double, C.c_double,
+                                                 C.c_double]
+        L.sem_shot_add_receiver.argtypes = [_VP, C.c_int, _D]
+        L.sem_shot_set_field.argtypes = [_VP, C.c_int, C.c_int, _D]
+        L.sem_shot_get_field.argtypes = [_VP, C.c_int, C.c_int, _D]
+        L.sem_batch_prepare.argtypes = [_VP, C.c_int, C.c_int, C.c_int]
+        L.sem_batch_run.argtypes = [_VP, C.c_int64]
+        L.sem_shot_progress.argtypes = [_VP, C.c_int, _I64, _D]
+        L.sem_shot_get_seismograms.argtypes = [_VP, C.c_int, _F]
+        L.sem_batch_config.argtypes = [_VP, _I64]
+        L.sem_batch_kernel_time.argtypes = [_VP, _D, _I64]
+        L.sem_reference_step_count.restype = C.c_int64
+        L.sem_reference_step_count.argtypes = [C.c_double, C.c_double, _D]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise SemError(lib().sem_last_error().decode())
+
+
+def _d(a):
+    return a.ctypes.data_as(_D)
+
+
+def _f(a):
+    return a.ctypes.data_as(_F)
+
+
+def _i(a):
+    return a.ctypes.data_as(_I32)
+
+
+def _l(a):
+    return a.ctypes.data_as(_I64)
+
+
+def device_count():
+    n = C.c_int(0)
+    lib().sem_device_count(C.byref(n))
+    return n.value
+
+
+def reference_step_count(duration, dt):
+    """Iterations of the reference's while (t < duration) loop and its dt
+    (Options.cpp:84-97, Main.cpp:45)."""
+    out = np.zeros(1)
+    n = lib().sem_reference_step_count(duration, dt, _d(out))
+    return int(n), float(out[0])
+
+
+def topology(dim, order, cells, nvert):
+    """Host-only gather map and colouring: (conn[E, nd], colour[E], info)."""
+    cells = np.ascontiguousarray(cells, dtype=np.int32)
+    nv = 4 if dim == 2 else 8
+    E = cells.size // nv
+    nd = (order + 1) ** dim
+    conn = np.zeros(E * nd, dtype=np.int32)
+    col = np.zeros(E, dtype=np.int32)
+    info = np.zeros(4, dtype=np.int64)
+    _check(lib().sem_topology_build(dim, order, E, nvert, _i(cells), _i(conn), _i(col), _l(info)))
+    return conn.reshape(E, nd), col, dict(G=int(info[0]), ncolours=int(info[1]),
+                                          faces=int(info[2]), edges=int(info[3]))
+
+
+class Mesh:
+    """Device-resident mesh: gather map, colouring, FP64-built geometry and
+    inverse mass (Mesh.cpp:31-51, :361-452; Order2Newmark.cpp:32-66)."""
+
+    def __init__(self, dim, order, physics, cells, coords, vertex_par, quirks=DEFAULT_QUIRKS,
+                 device=0):
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        par = np.ascontiguousarray(vertex_par, dtype=np.float64)
+        nv = 4 if dim == 2 else 8
+        self.dim, self.order, self.physics, self.quirks = dim, order, physics, quirks
+        self.E = cells.size // nv
+        self.V = coords.size // dim
+        self._h = _VP()
+        _check(lib().sem_mesh_create(C.byref(self._h), device, dim, order, physics, self.E, self.V,
+                                     _i(cells), _d(coords), _d(par), quirks))
+        info = np.zeros(8, dtype=np.int64)
+        lib().sem_mesh_info(self._h, _l(info))
+        self.G, self.ncolours, self.c, self.nd = (int(x) for x in info[1:5])
+        self.affine = bool(info[5])
+        self.device_bytes = int(info[6])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sem_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def conn(self):
+        out = np.zeros(self.E * self.nd, dtype=np.int32)
+        _check(lib().sem_mesh_get_conn(self._h, _i(out)))
+        return out.reshape(self.E, self.nd)
+
+    def colours(self):
+        out = np.zeros(self.E, dtype=np.int32)
+        _check(lib().sem_mesh_get_colours(self._h, _i(out)))
+        return out
+
+    def inverse_mass(self):
+        out = np.zeros(self.G, dtype=np.float32)
+        _check(lib().sem_mesh_get_inverse_mass(self._h, _f(out)))
+        return out
+
+    def node_coords(self):
+        out = np.zeros(self.G * self.dim)
+        _check(lib().sem_mesh_get_node_coords(self._h, _d(out)))
+        return out.reshape(self.G, self.dim)
+
+    def set_dirichlet(self, mask):
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        _check(lib().sem_mesh_set_dirichlet(self._h, m.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+    def apply_stiffness(self, u, path=PATH_FUSED):
+        u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1)
+        out = np.zeros(self.c * self.G, dtype=np.float32)
+        _check(lib().sem_mesh_apply_stiffness(self._h, path, _f(u), _f(out)))
+        return out.reshape(self.c, self.G)
+
+    def check_colouring(self):
+        v = np.zeros(1, dtype=np.int64)
+        _check(lib().sem_mesh_check_colouring(self._h, _l(v)))
+        return int(v[0])
+
+
+class Batch:
+    """Shots advanced together (Problem::initializeElements source/receiver
+    attach, Problem.cpp:34-142; the step loop of Main.cpp:45-66)."""
+
+    def __init__(self, device=0):
+        self._h = _VP()
+        _check(lib().sem_batch_create(C.byref(self._h), device))
+        self.meshes = []
+        self.shots = []  # (mesh, nsteps, nrec)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sem_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_shot(self, mesh, nsteps, dt):
+        k = C.c_int(0)
+        _check(lib().sem_batch_add_shot(self._h, mesh.handle, nsteps, dt, C.byref(k)))
+        self.meshes.append(mesh)
+        self.shots.append([mesh, nsteps, 0])
+        return k.value
+
+    def add_source(self, shot, loc, amp, f0, t0, direction=None):
+        m = self.shots[shot][0]
+        d = np.zeros(3)
+        if direction is None:
+            d[0] = 1.0
+        else:
+            d[: len(direction)] = direction
+        _check(lib().sem_shot_add_source(self._h, shot, _d(np.asarray(loc, dtype=np.float64)),
+                                         amp, f0, t0, _d(d)))
+
+    def add_moment_source(self, shot, loc, M, amp, f0, t0):
+        _check(lib().sem_shot_add_moment_source(
+            self._h, shot, _d(np.asarray(loc, dtype=np.float64)),
+            _d(np.ascontiguousarray(M, dtype=np.float64)), amp, f0, t0))
+
+    def add_receiver(self, shot, loc):
+        _check(lib().sem_shot_add_receiver(self._h, shot, _d(np.asarray(loc, dtype=np.float64))))
+        self.shots[shot][2] += 1
+
+    def set_field(self, shot, which, vals):
+        v = np.ascontiguousarray(vals, dtype=np.float64).reshape(-1)
+        _check(lib().sem_shot_set_field(self._h, shot, FIELD[which], _d(v)))
+
+    def get_field(self, shot, which):
+        m = self.shots[shot][0]
+        out = np.zeros(m.c * m.G)
+        _check(lib().sem_shot_get_field(self._h, shot, FIELD[which], _d(out)))
+        return out.reshape(m.c, m.G)
+
+    def prepare(self, path=PATH_FUSED, elems_per_block=0, shots_per_group=0):
+        _check(lib().sem_batch_prepare(self._h, path, elems_per_block, shots_per_group))
+
+    def run(self, nsteps):
+        _check(lib().sem_batch_run(self._h, nsteps))
+
+    def progress(self, shot):
+        s, t = np.zeros(1, dtype=np.int64), np.zeros(1)
+        _check(lib().sem_shot_progress(self._h, shot, _l(s), _d(t)))
+        return int(s[0]), float(t[0])
+
+    def seismograms(self, shot):
+        m, nsteps, nrec = self.shots[shot]
+        out = np.zeros(max(nsteps * nrec * m.c, 1), dtype=np.float32)
+        _check(lib().sem_shot_get_seismograms(self._h, shot, _f(out)))
+        return out[: nsteps * nrec * m.c].reshape(nsteps, nrec, m.c)
+
+    def config(self):
+        c = np.zeros(4, dtype=np.int64)
+        _check(lib().sem_batch_config(self._h, _l(c)))
+        return dict(elems_per_block=int(c[0]), shots_per_group=int(c[1]),
+                    block_steps=int(c[2]), groups=int(c[3]))
+
+    def kernel_time(self):
+        ms, n = np.zeros(1), np.zeros(1, dtype=np.int64)
+        _check(lib().sem_batch_kernel_time(self._h, _d(ms), _l(n)))
+        return float(ms[0]), int(n[0])
+
+
+class Order2Newmark:
+    """One shot driven like the reference's Problem/Order2Newmark loop
+    (Main.cpp:45-66): explicit Newmark with the reference's update order, the
+    stiffness in FP32 on the device and Kahan-compensated FP32 u/v."""
+
+    def __init__(self, mesh, dt, nsteps, path=PATH_FUSED, device=0):
+        self.mesh, self.dt, self.nsteps, self.path = mesh, dt, nsteps, path
+        self.batch = Batch(device)
+        self.shot = self.batch.add_shot(mesh, nsteps, dt)
+        self._prepared = False
+
+    def add_source(self, loc, amp, f0, t0, direction=None):
+        self.batch.add_source(self.shot, loc, amp, f0, t0, direction)
+
+    def add_moment_source(self, loc, M, amp, f0, t0):
+        self.batch.add_moment_source(self.shot, loc, M, amp, f0, t0)
+
+    def add_receiver(self, loc):
+        self.batch.add_receiver(self.shot, loc)
+
+    def set_field(self, which, vals):
+        self.batch.set_field(self.shot, which, vals)
+
+    def get_field(self, which):
+        return self.batch.get_field(self.shot, which)
+
+    def initializeGlobalDofs(self):
+        if not self._prepared:
+            self.batch.prepare(self.path)
+            self._prepared = True
+
+    def takeTimeStep(self, nsteps=1):
+        self.initializeGlobalDofs()
+        self.batch.run(nsteps)
+
+    def seismograms(self):
+        return self.batch.seismograms(self.shot)
+
+    def time(self):
+        return self.batch.progress(self.shot)[1]

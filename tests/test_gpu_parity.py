@@ -1,0 +1,205 @@
+"""GPU parity of the HIP path against the FP64 oracle, through the C ABI.
+
+Bars: gather maps and colourings bit-exact; FP32 stiffness and mass within
+FP32 rounding of FP64 (relative 2e-6); receiver seismograms and final
+wavefields within relative L2 1e-5 of the FP64 oracle; batched results
+identical (bitwise) to the same shot run alone; graph replay identical to
+eager launches."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from salvus_amd import meshgen, sem
+from tests import problems as pb
+from tests import refmeshes as rm
+
+pytestmark = pytest.mark.gpu
+
+TOL_OP = 2e-6    # one FP32 operator application vs FP64
+TOL_RUN = 1e-5   # receiver / wavefield relative L2 over a run (BASELINE metric bar)
+
+
+def oracle_for(P, quirks=po.DEFAULT_QUIRKS):
+    S = po.Sim(P["dim"], P["order"], P["physics"], P["cells"], P["coords"], P["par"], quirks)
+    return S
+
+
+def mesh_for(P, quirks=sem.DEFAULT_QUIRKS):
+    return sem.Mesh(P["dim"], P["order"], P["physics"], P["cells"], P["coords"], P["par"], quirks)
+
+
+CASES = [(2, 0), (2, 1), (3, 0), (3, 1)]
+
+
+@pytest.mark.parametrize("dim,physics", CASES)
+def test_mesh_maps_bit_exact(dim, physics):
+    P = pb.make(dim, physics, 5 if dim == 3 else 9, order=4)
+    M = mesh_for(P)
+    S = oracle_for(P)
+    np.testing.assert_array_equal(M.conn(), S.conn())
+    ocol, _ = po.color_elements(dim, P["cells"], len(P["coords"]))
+    np.testing.assert_array_equal(M.colours(), ocol)
+    assert M.check_colouring() == 0
+    assert M.affine
+    mi = M.inverse_mass().astype(np.float64)
+    ref = S.inverse_mass()
+    assert np.max(np.abs(mi - ref) / np.abs(ref)) < 1e-6
+
+
+@pytest.mark.parametrize("dim,physics", CASES)
+@pytest.mark.parametrize("order", [3, 4])
+@pytest.mark.parametrize("distorted", [False, True])
+def test_stiffness_operator(dim, physics, order, distorted):
+    if distorted and dim == 2:
+        pytest.skip("non-affine quads: reference applies J^-1 at output nodes (Q8) with a "
+                    "swapped-coordinate Jacobian (Q21); product uses the consistent form")
+    P = pb.make(dim, physics, 4 if dim == 3 else 7, order=order, seed=order, distorted=distorted)
+    M = mesh_for(P)
+    S = oracle_for(P)
+    assert M.affine == (not distorted)
+    rng = np.random.default_rng(11)
+    xyz = S.node_coords()
+    u = np.stack([np.sin(0.05 * xyz @ rng.normal(size=dim)) + 0.1 * rng.normal(size=S.G)
+                  for _ in range(S.c)])
+    ref = S.apply_stiffness(u)
+    for path in (sem.PATH_FUSED, sem.PATH_PLAIN):
+        ku = M.apply_stiffness(u, path).astype(np.float64)
+        assert pb.rel_l2(ku, ref) < TOL_OP, (path, pb.rel_l2(ku, ref))
+
+
+def _run_both(P, path, quirks=po.DEFAULT_QUIRKS, init=None):
+    S = oracle_for(P, quirks)
+    M = mesh_for(P, quirks)
+    src = P["src"]
+    S.add_source(src["loc"], src["amp"], src["f0"], src["t0"], src["dir"])
+    for r in P["recs"]:
+        S.add_receiver(r)
+    prob = sem.Order2Newmark(M, P["dt"], P["nsteps"], path=path)
+    prob.add_source(src["loc"], src["amp"], src["f0"], src["t0"], src["dir"])
+    for r in P["recs"]:
+        prob.add_receiver(r)
+    if init is not None:
+        S.set_field("u", init)
+        prob.set_field("u", init)
+    seis_ref = S.run(P["dt"], P["nsteps"])
+    prob.takeTimeStep(P["nsteps"])
+    return (prob.seismograms(), seis_ref, prob.get_field("u"), S.get_field("u"), prob, S)
+
+
+@pytest.mark.parametrize("dim,physics", CASES)
+@pytest.mark.parametrize("path", [sem.PATH_FUSED, sem.PATH_PLAIN])
+def test_run_receivers_and_wavefield(dim, physics, path):
+    P = pb.make(dim, physics, 5 if dim == 3 else 10, order=4, seed=3, nsteps=300)
+    seis, seis_ref, u, u_ref, prob, S = _run_both(P, path)
+    assert np.abs(seis_ref).max() > 0
+    e_seis, e_u = pb.rel_l2(seis, seis_ref), pb.rel_l2(u, u_ref)
+    print("dim %d physics %d path %d: receivers rel-L2 %.3e, wavefield %.3e" % (dim, physics, path, e_seis, e_u))
+    assert e_seis < TOL_RUN and e_u < TOL_RUN
+    steps, t = prob.batch.progress(prob.shot)
+    assert steps == P["nsteps"] and t == pytest.approx(S.time(), rel=1e-15)
+
+
+def test_distorted_hex_run():
+    P = pb.make(3, 1, 4, order=4, seed=5, distorted=True, nsteps=200)
+    seis, seis_ref, u, u_ref, _, _ = _run_both(P, sem.PATH_FUSED)
+    assert pb.rel_l2(seis, seis_ref) < TOL_RUN and pb.rel_l2(u, u_ref) < TOL_RUN
+
+
+def test_newmark3d_quirk_matches_oracle():
+    P = pb.make(3, 1, 4, order=3, seed=6, nsteps=120)
+    q = po.QUIRK_NEWMARK3D | po.QUIRK_E3D_SOURCE
+    seis, seis_ref, u, u_ref, _, _ = _run_both(P, sem.PATH_FUSED, quirks=q)
+    assert pb.rel_l2(u, u_ref) < TOL_RUN
+    assert np.abs(u[2]).max() == 0.0  # z never advances under the reference's list
+
+
+def test_quad_eigenfunction_on_gpu():
+    """test_Scalar_2D.cpp:238-308 through the HIP path (FP32)."""
+    cells, coords = rm.quad_4x4()
+    L, vp = 1e5, 5800.0
+    M = sem.Mesh(2, 3, sem.ACOUSTIC, cells, coords, np.full(len(cells) * 4, vp))
+    xyz = M.node_coords()
+    un = np.prod(np.sin(np.pi / L * (xyz - L)), axis=1)
+    M.set_dirichlet(np.any(np.isclose(xyz, 0, atol=1e-6 * L) | np.isclose(xyz, L, atol=1e-6 * L), axis=1))
+    nsteps = 2439
+    prob = sem.Order2Newmark(M, 1e-2, nsteps)
+    prob.set_field("u", un)
+    worst, t = 0.0, 0.0
+    for k in range(nsteps):
+        prob.takeTimeStep(1)
+        t += 1e-2
+        exact = np.cos(np.pi / L * np.sqrt(2) * t * vp) * un
+        worst = max(worst, np.abs(exact - prob.get_field("u")[0]).max())
+    assert worst <= 0.001288 * 1.01
+
+
+def test_batched_identical_to_alone():
+    """Shots sharing a mesh (interleaved group) and a shot on another mesh,
+    advanced in one batch, equal the same shots run alone, bit for bit."""
+    P1 = pb.make(3, 0, 4, order=4, seed=7, nsteps=80)
+    P2 = pb.make(3, 0, 5, order=4, seed=8, nsteps=60)
+    M1, M2 = mesh_for(P1), mesh_for(P2)
+    locs = [P1["src"]["loc"] * f for f in (1.0, 0.8, 1.1)]
+
+    def build(shots):
+        B = sem.Batch()
+        ids = []
+        for (M, P, loc) in shots:
+            k = B.add_shot(M, P["nsteps"], P["dt"])
+            B.add_source(k, loc, P["src"]["amp"], P["src"]["f0"], P["src"]["t0"])
+            for r in P["recs"]:
+                B.add_receiver(k, r)
+            ids.append(k)
+        return B, ids
+
+    mix = [(M1, P1, l) for l in locs] + [(M2, P2, P2["src"]["loc"])]
+    B, ids = build(mix)
+    B.prepare(sem.PATH_FUSED, 0, 4)
+    assert B.config()["groups"] == 2
+    B.run(80)
+    for (M, P, loc), k in zip(mix, ids):
+        A, (a,) = build([(M, P, loc)])
+        A.prepare(sem.PATH_FUSED, B.config()["elems_per_block"], 1)
+        A.run(80)
+        np.testing.assert_array_equal(A.seismograms(a), B.seismograms(k))
+        np.testing.assert_array_equal(A.get_field(a, "u"), B.get_field(k, "u"))
+
+
+def test_graph_replay_identical_to_eager(monkeypatch):
+    P = pb.make(2, 1, 8, order=4, seed=9, nsteps=160)
+    out = []
+    for no_graph in ("1", "0", "0"):
+        monkeypatch.setenv("SEM_NO_GRAPH", no_graph)
+        M = mesh_for(P)
+        prob = sem.Order2Newmark(M, P["dt"], P["nsteps"])
+        prob.add_source(P["src"]["loc"], P["src"]["amp"], P["src"]["f0"], P["src"]["t0"], P["src"]["dir"])
+        for r in P["recs"]:
+            prob.add_receiver(r)
+        prob.initializeGlobalDofs()
+        for _ in range(4):  # several replays with host work between them
+            prob.takeTimeStep(40)
+            _ = prob.get_field("v")
+        out.append((prob.seismograms(), prob.get_field("u")))
+    for s, u in out[1:]:
+        np.testing.assert_array_equal(s, out[0][0])
+        np.testing.assert_array_equal(u, out[0][1])
+
+
+def test_moment_tensor_source_matches_oracle():
+    P = pb.make(3, 1, 4, order=4, seed=10, nsteps=150)
+    M = mesh_for(P)
+    S = oracle_for(P)
+    Mt = np.array([[1.0, 0.2, -0.4], [0.2, -0.6, 0.3], [-0.4, 0.3, 0.5]]) * 1e6
+    src = P["src"]
+    S.add_moment_source(src["loc"], Mt, 1.0, src["f0"], src["t0"])
+    for r in P["recs"]:
+        S.add_receiver(r)
+    prob = sem.Order2Newmark(M, P["dt"], P["nsteps"])
+    prob.add_moment_source(src["loc"], Mt, 1.0, src["f0"], src["t0"])
+    for r in P["recs"]:
+        prob.add_receiver(r)
+    ref = S.run(P["dt"], P["nsteps"])
+    prob.takeTimeStep(P["nsteps"])
+    assert pb.rel_l2(prob.seismograms(), ref) < TOL_RUN
