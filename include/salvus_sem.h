@@ -74,8 +74,9 @@ int sem_mesh_create(sem_mesh **mesh, int device, int dim, int order, int physics
                     int64_t num_cells, int64_t num_verts, const int32_t *cells,
                     const double *coords, const double *vertex_par, int quirks);
 int sem_mesh_destroy(sem_mesh *mesh);
-/* info[0..7] = {E, G (unique nodes), ncolours, components, nodes/element,
- *               affine (1/0), device bytes held, dim} */
+/* info[0..9] = {E, G (unique nodes), ncolours, components, nodes/element,
+ *               affine (1/0), device bytes held, dim, tiles (0 = element
+ *               kernel), tile shape Tx*10000 + Ty*100 + Tz} */
 int sem_mesh_info(const sem_mesh *mesh, int64_t *info);
 /* Gather map conn[e*nd + p] = global DOF, element order as given
  * (DMPlexVecGetClosure + ClsMap, Problem.cpp:302-320). */

@@ -81,12 +81,15 @@ struct Group {
   SrcDev *d_src = nullptr;
   float *d_stf = nullptr, *d_load = nullptr;
   int nsrc = 0;
+  bool tiled = false;  // tile kernel; fields in the mesh's tile-major numbering
 };
 
 struct LaunchSet {
   int dim, order, physics, nd, c, colour;
   bool affine;
+  bool tile;
   int nitems;
+  int ntn_max, sub_max, emax;
   WorkItem *d_items;
 };
 
@@ -113,6 +116,7 @@ struct sem_batch {
   std::vector<LaunchSet> launches;
   std::vector<RecSet> recsets;
   float *d_rec_w = nullptr;
+  uint32_t *d_rec_nodes = nullptr;
   float *d_seis = nullptr;
   int64_t seis_total = 0;
   float *d_uloc = nullptr, *d_aloc = nullptr;  // plain path scratch
@@ -209,6 +213,87 @@ static int ensure_pointwise(sem_mesh *m) {
   return 0;
 }
 
+// Default tile shape per dimension/physics (overridable with SEM_TILE=x,y[,z]).
+static void default_tile(int dim, int physics, int *T) {
+  if (dim == 2) { T[0] = 8; T[1] = 8; T[2] = 1; }
+  else if (physics == SEM_ACOUSTIC) { T[0] = 4; T[1] = 4; T[2] = 2; }
+  else { T[0] = 4; T[1] = 2; T[2] = 2; }
+  const char *env = getenv("SEM_TILE");
+  if (env && *env) {
+    int a = 0, b = 0, c = 1;
+    if (sscanf(env, "%d,%d,%d", &a, &b, &c) >= 2 && a > 0 && b > 0 && c > 0) {
+      T[0] = a; T[1] = b; T[2] = dim == 3 ? c : 1;
+    }
+  }
+}
+
+static int upload_tile_words(sem_mesh *m) {
+  const TileSet &ts = m->tiles;
+  if (!m->d_tile_words) SEM_TRY(dalloc(&m->d_tile_words, ts.words.size(), &m->bytes));
+  HIPCHK(hipMemcpyAsync(m->d_tile_words, ts.words.data(), ts.words.size() * 4, hipMemcpyHostToDevice, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+// Builds and uploads the tile path of an affine, logically Cartesian mesh.
+static int setup_tiles(sem_mesh *m) {
+  const char *off = getenv("SEM_NO_TILES");
+  if (!m->affine || (off && off[0] == '1')) return 0;
+  int T[3];
+  default_tile(m->dim, m->physics, T);
+  // shrink the tile until its LDS footprint fits 64 KiB
+  for (;;) {
+    const int N = m->order;
+    int ntn = (T[0] * N + 1) * (T[1] * N + 1) * (m->dim == 3 ? T[2] * N + 1 : 1);
+    int sub = ((T[0] + 1) / 2) * ((T[1] + 1) / 2) * (m->dim == 3 ? (T[2] + 1) / 2 : 1);
+    sub = std::max(1, std::min(sub, 512 / m->nd));
+    if (tile_lds_bytes(m->dim, m->physics, m->nd, m->c, ntn, sub, T[0] * T[1] * T[2]) <= 64 * 1024 || (T[0] == 1 && T[1] == 1 && T[2] == 1)) break;
+    int a = (T[0] >= T[1] && T[0] >= T[2]) ? 0 : (T[1] >= T[2] ? 1 : 2);
+    T[a] = std::max(1, T[a] / 2);
+  }
+  if (build_tiles(m, T[0], T[1], T[2], m->tiles) != 0) {
+    m->tiles = TileSet();
+    return 0;  // not eligible: the element kernel is used
+  }
+  TileSet &ts = m->tiles;
+  const int dim = m->dim, nv = m->nv, npar = ref_npar(dim, m->physics), kp = kernel_npar(dim, m->physics);
+  const int64_t E = m->E;
+  SEM_TRY(dalloc(&m->d_tile_info, ts.info.size(), &m->bytes));
+  HIPCHK(hipMemcpyAsync(m->d_tile_info, ts.info.data(), ts.info.size() * sizeof(TileInfo), hipMemcpyHostToDevice, m->stream));
+  SEM_TRY(upload_tile_words(m));
+  // FP64 vertices in tile element order -> affine factors on the device
+  std::vector<double> tv(E * nv * dim);
+  std::vector<float> tp(E * nv * kp);
+  for (int64_t i = 0; i < E; ++i) {
+    int64_t e = ts.elem_order[i];
+    std::copy_n(&m->vtx[e * nv * dim], nv * dim, &tv[i * nv * dim]);
+    for (int k = 0; k < nv; ++k) {
+      const double *src = &m->vpar[(e * nv + k) * npar];
+      float *dst = &tp[(i * nv + k) * kp];
+      if (m->physics == SEM_ACOUSTIC) dst[0] = (float)src[0];
+      else if (dim == 2) for (int q = 0; q < 4; ++q) dst[q] = (float)src[1 + q];
+      else for (int q = 0; q < 6; ++q) dst[q] = (float)src[q];
+    }
+  }
+  double *d_tv = nullptr;
+  SEM_TRY(dalloc(&d_tv, tv.size()));
+  int rc = hip_check(hipMemcpyAsync(d_tv, tv.data(), tv.size() * 8, hipMemcpyHostToDevice, m->stream), "upload");
+  if (!rc) rc = dalloc(&m->d_tgeo_e, (size_t)E * affine_geo_count(dim, m->physics), &m->bytes);
+  if (!rc) rc = launch_affine_geometry(dim, m->physics, E, d_tv, m->d_tgeo_e, m->stream);
+  if (!rc) rc = dalloc(&m->d_tpar_e, tp.size(), &m->bytes);
+  if (!rc) rc = hip_check(hipMemcpyAsync(m->d_tpar_e, tp.data(), tp.size() * 4, hipMemcpyHostToDevice, m->stream), "upload");
+  // inverse mass in the internal numbering
+  int32_t *d_to = nullptr;
+  if (!rc) rc = dalloc(&d_to, (size_t)m->G);
+  if (!rc) rc = hip_check(hipMemcpyAsync(d_to, ts.to_internal.data(), m->G * 4, hipMemcpyHostToDevice, m->stream), "upload");
+  if (!rc) rc = dalloc(&m->d_mi_int, (size_t)m->G, &m->bytes);
+  if (!rc) rc = launch_permute_f32(m->G, d_to, m->d_mi, m->d_mi_int, m->stream);
+  if (!rc) rc = hip_check(hipStreamSynchronize(m->stream), "sync");
+  dfree(d_tv);
+  dfree(d_to);
+  return rc;
+}
+
 static void mesh_free(sem_mesh *m) {
   if (!m) return;
   (void)hipSetDevice(m->device);
@@ -220,6 +305,11 @@ static void mesh_free(sem_mesh *m) {
   dfree(m->d_mi);
   dfree(m->d_vtx);
   dfree(m->d_vpar);
+  dfree(m->d_tile_info);
+  dfree(m->d_tile_words);
+  dfree(m->d_tgeo_e);
+  dfree(m->d_tpar_e);
+  dfree(m->d_mi_int);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
 }
@@ -394,6 +484,7 @@ int sem_mesh_create(sem_mesh **out, int device, int dim, int order, int physics,
     if (rc) return fail(rc);
   }
   if (!affine && (rc = ensure_pointwise(m))) return fail(rc);
+  if ((rc = setup_tiles(m))) return fail(rc);
   *out = m;
   return 0;
 }
@@ -406,6 +497,8 @@ int sem_mesh_destroy(sem_mesh *m) {
 int sem_mesh_info(const sem_mesh *m, int64_t *info) {
   info[0] = m->E; info[1] = m->G; info[2] = m->ncol; info[3] = m->c;
   info[4] = m->nd; info[5] = m->affine; info[6] = (int64_t)m->bytes; info[7] = m->dim;
+  info[8] = m->tiles.valid ? m->tiles.ntiles : 0;
+  info[9] = m->tiles.valid ? m->tiles.T[0] * 10000 + m->tiles.T[1] * 100 + m->tiles.T[2] : 0;
   return 0;
 }
 
@@ -448,7 +541,19 @@ int sem_mesh_get_node_coords(const sem_mesh *m, double *xyz) {
 int sem_mesh_set_dirichlet(sem_mesh *m, const uint8_t *mask) {
   HIPCHK(hipSetDevice(m->device));
   m->dirichlet.assign(mask, mask + m->G);
-  return upload_conn(m);
+  SEM_TRY(upload_conn(m));
+  if (m->tiles.valid) {
+    const uint32_t flags = kFlagFirst | kFlagLast;
+    TileSet &ts = m->tiles;
+    std::vector<uint8_t> dint(m->G, 0);
+    for (int64_t g = 0; g < m->G; ++g) dint[ts.to_internal[g]] = m->dirichlet[g];
+    for (auto &w : ts.words) {
+      uint32_t node = w & kNodeMask;
+      w = node | (w & flags) | (dint[node] ? kFlagDirichlet : 0u);
+    }
+    SEM_TRY(upload_tile_words(m));
+  }
+  return 0;
 }
 
 int sem_mesh_check_colouring(sem_mesh *m, int64_t *violations) {
@@ -587,6 +692,7 @@ static void batch_release(sem_batch *b) {
   dfree(b->d_groups);
   dfree(b->d_steps);
   dfree(b->d_rec_w);
+  dfree(b->d_rec_nodes);
   dfree(b->d_seis);
   dfree(b->d_uloc);
   dfree(b->d_aloc);
@@ -690,6 +796,14 @@ int sem_shot_add_receiver(sem_batch *b, int shot, const double *loc) {
   return 0;
 }
 
+// Position of DMPlex DOF entry i (= comp*G + g) in a group's field arrays.
+static inline int64_t field_pos(const Group &g, int64_t i) {
+  if (!g.tiled) return i;
+  const sem_mesh *m = g.mesh;
+  const int64_t c = i / m->G, node = i - c * m->G;
+  return c * m->G + m->tiles.to_internal[node];
+}
+
 static float *field_base(sem_batch *b, const Shot &s, int which) {
   Group &g = b->groups[s.group];
   const size_t len = (size_t)s.mesh->c * s.mesh->G * g.S;
@@ -712,7 +826,7 @@ int sem_shot_set_field(sem_batch *b, int shot, int which, const double *vals) {
   float *base = field_base(b, *s, which);
   HIPCHK(hipMemcpyAsync(buf.data(), base, buf.size() * 4, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
-  for (size_t i = 0; i < len; ++i) buf[i * g.S + s->slot] = (float)vals[i];
+  for (size_t i = 0; i < len; ++i) buf[field_pos(g, i) * g.S + s->slot] = (float)vals[i];
   HIPCHK(hipMemcpyAsync(base, buf.data(), buf.size() * 4, hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
@@ -732,7 +846,7 @@ int sem_shot_get_field(sem_batch *b, int shot, int which, double *vals) {
   std::vector<float> buf(len * g.S);
   HIPCHK(hipMemcpyAsync(buf.data(), field_base(b, *s, which), buf.size() * 4, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
-  for (size_t i = 0; i < len; ++i) vals[i] = buf[i * g.S + s->slot];
+  for (size_t i = 0; i < len; ++i) vals[i] = buf[field_pos(g, i) * g.S + s->slot];
   return 0;
 }
 
@@ -751,41 +865,71 @@ static int default_epb(int nd) {
 static double time_shape(sem_batch *b, int epb, int reps);
 
 static int build_plan(sem_batch *b) {
-  // work items per (kernel key, colour)
-  std::map<std::tuple<int, int, int, int>, std::vector<int>> keys;  // key -> groups
+  // work items per (kernel key, colour); key = (tile?, dim, order, physics, affine)
+  std::map<std::tuple<int, int, int, int, int>, std::vector<int>> keys;
   for (int gi = 0; gi < (int)b->groups.size(); ++gi) {
     sem_mesh *m = b->groups[gi].mesh;
-    keys[std::make_tuple(m->dim, m->order, m->physics, (int)m->affine)].push_back(gi);
+    keys[std::make_tuple((int)b->groups[gi].tiled, m->dim, m->order, m->physics, (int)m->affine)]
+        .push_back(gi);
   }
   for (auto &l : b->launches) dfree(l.d_items);
   b->launches.clear();
   for (auto &kv : keys) {
+    const bool tile = std::get<0>(kv.first) != 0;
     int maxcol = 0;
-    for (int gi : kv.second) maxcol = std::max(maxcol, b->groups[gi].mesh->ncol);
+    for (int gi : kv.second) {
+      sem_mesh *m = b->groups[gi].mesh;
+      maxcol = std::max(maxcol, tile ? (1 << m->dim) : m->ncol);
+    }
     for (int c = 0; c < maxcol; ++c) {
       std::vector<WorkItem> items;
+      int ntn_max = 0, sub_max = 0, emax = 0;
       for (int gi : kv.second) {
         sem_mesh *m = b->groups[gi].mesh;
-        if (c >= m->ncol) continue;
-        for (int64_t f = m->col_start[c]; f < m->col_start[c + 1]; f += b->epb) {
-          WorkItem w;
-          w.group = gi;
-          w.first = (int32_t)f;
-          w.count = (int32_t)std::min<int64_t>(b->epb, m->col_start[c + 1] - f);
-          w.pad = 0;
-          items.push_back(w);
+        if (tile) {
+          const TileSet &ts = m->tiles;
+          if (c >= (int)ts.colour_start.size() - 1) continue;
+          for (int64_t k = ts.colour_start[c]; k < ts.colour_start[c + 1]; ++k) {
+            const int32_t t = ts.by_colour[k];
+            const TileInfo &ti = ts.info[t];
+            WorkItem w;
+            w.group = gi;
+            w.first = t;
+            w.count = ti.ex * ti.ey * ti.ez;
+            w.pad = 0;
+            items.push_back(w);
+          }
+          const int N = m->order;
+          ntn_max = std::max(ntn_max, (ts.T[0] * N + 1) * (ts.T[1] * N + 1) * (m->dim == 3 ? ts.T[2] * N + 1 : 1));
+          int sub = ((ts.T[0] + 1) / 2) * ((ts.T[1] + 1) / 2) * (m->dim == 3 ? (ts.T[2] + 1) / 2 : 1);
+          sub_max = std::max(sub_max, std::max(1, std::min(sub, 512 / m->nd)));
+          emax = std::max(emax, ts.T[0] * ts.T[1] * (m->dim == 3 ? ts.T[2] : 1));
+        } else {
+          if (c >= m->ncol) continue;
+          for (int64_t f = m->col_start[c]; f < m->col_start[c + 1]; f += b->epb) {
+            WorkItem w;
+            w.group = gi;
+            w.first = (int32_t)f;
+            w.count = (int32_t)std::min<int64_t>(b->epb, m->col_start[c + 1] - f);
+            w.pad = 0;
+            items.push_back(w);
+          }
         }
       }
       if (items.empty()) continue;
       LaunchSet L;
-      L.dim = std::get<0>(kv.first);
-      L.order = std::get<1>(kv.first);
-      L.physics = std::get<2>(kv.first);
-      L.affine = std::get<3>(kv.first) != 0;
+      L.tile = tile;
+      L.dim = std::get<1>(kv.first);
+      L.order = std::get<2>(kv.first);
+      L.physics = std::get<3>(kv.first);
+      L.affine = std::get<4>(kv.first) != 0;
       L.nd = L.dim == 2 ? (L.order + 1) * (L.order + 1) : (L.order + 1) * (L.order + 1) * (L.order + 1);
       L.c = L.physics == SEM_ACOUSTIC ? 1 : L.dim;
       L.colour = c;
       L.nitems = (int)items.size();
+      L.ntn_max = ntn_max;
+      L.sub_max = sub_max;
+      L.emax = emax;
       SEM_TRY(dalloc(&L.d_items, items.size()));
       HIPCHK(hipMemcpyAsync(L.d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, b->stream));
       b->launches.push_back(L);
@@ -804,6 +948,13 @@ static int enqueue_fused_step(sem_batch *b, int koff, int ku_only) {
     a.epb = b->epb;
     a.koff = koff;
     a.ku_only = ku_only;
+    a.ntn_max = L.ntn_max;
+    a.sub_max = L.sub_max;
+    if (L.tile) a.epb = L.emax;
+    auto go = [&]() {
+      return L.tile ? launch_tile(L.dim, L.order, L.physics, L.nitems, L.nd, L.c, a, b->stream)
+                    : launch_fused(L.dim, L.order, L.physics, L.affine, L.nitems, b->epb, L.nd, L.c, a, b->stream);
+    };
     if (b->timing && !ku_only) {
       hipEvent_t e0, e1;
       HIPCHK(hipEventCreate(&e0));
@@ -811,10 +962,10 @@ static int enqueue_fused_step(sem_batch *b, int koff, int ku_only) {
       b->ev.push_back(e0);
       b->ev.push_back(e1);
       HIPCHK(hipEventRecord(e0, b->stream));
-      SEM_TRY(launch_fused(L.dim, L.order, L.physics, L.affine, L.nitems, b->epb, L.nd, L.c, a, b->stream));
+      SEM_TRY(go());
       HIPCHK(hipEventRecord(e1, b->stream));
     } else {
-      SEM_TRY(launch_fused(L.dim, L.order, L.physics, L.affine, L.nitems, b->epb, L.nd, L.c, a, b->stream));
+      SEM_TRY(go());
     }
   }
   if (ku_only) return 0;
@@ -823,6 +974,7 @@ static int enqueue_fused_step(sem_batch *b, int koff, int ku_only) {
     ra.groups = b->d_groups;
     ra.rec_group = R.d_group;
     ra.recs = R.d_recs;
+    ra.nodes = b->d_rec_nodes;
     ra.weights = b->d_rec_w;
     ra.out = b->d_seis;
     ra.nrec = R.nrec;
@@ -880,6 +1032,7 @@ static int enqueue_plain_step(sem_batch *b) {
     ra.groups = b->d_groups;
     ra.rec_group = R.d_group;
     ra.recs = R.d_recs;
+    ra.nodes = b->d_rec_nodes;
     ra.weights = b->d_rec_w;
     ra.out = b->d_seis;
     ra.nrec = R.nrec;
@@ -939,12 +1092,14 @@ int sem_batch_prepare(sem_batch *b, int path, int epb, int spg) {
   auto fail = [&](int r) { batch_release(b); return r; };
   // ---- per-group device state
   std::vector<float> rec_w;
+  std::vector<uint32_t> rec_nodes;
   std::vector<std::vector<std::pair<RecDev, int>>> recs_by_key;  // per (C, nd) key
   std::map<int, int> rec_key;
   int64_t seis = 0;
   for (int gi = 0; gi < (int)b->groups.size(); ++gi) {
     Group &g = b->groups[gi];
     sem_mesh *m = g.mesh;
+    g.tiled = path == SEM_PATH_FUSED && m->tiles.valid;
     const size_t len = (size_t)m->c * m->G * g.S;
     if ((rc = dalloc(&g.fields, 6 * len))) return fail(rc);
     std::vector<float> buf(len);
@@ -954,7 +1109,7 @@ int sem_batch_prepare(sem_batch *b, int path, int epb, int spg) {
         for (int sl = 0; sl < g.S; ++sl) {
           const Shot &s = b->shots[g.shots[sl]];
           if (s.init[which].empty()) continue;
-          for (size_t i = 0; i < (size_t)m->c * m->G; ++i) buf[i * g.S + sl] = (float)s.init[which][i];
+          for (size_t i = 0; i < (size_t)m->c * m->G; ++i) buf[field_pos(g, i) * g.S + sl] = (float)s.init[which][i];
         }
       if ((rc = hip_check(hipMemcpyAsync(g.fields + which * len, buf.data(), len * 4, hipMemcpyHostToDevice, b->stream), "upload"))) return fail(rc);
     }
@@ -972,7 +1127,7 @@ int sem_batch_prepare(sem_batch *b, int path, int epb, int spg) {
         std::vector<double> ld;
         if ((rc = source_load(m, sp, &e, ld))) return fail(rc);
         SrcDev d;
-        d.elem = m->inv_perm[e];
+        d.elem = g.tiled ? m->tiles.elem_pos[e] : m->inv_perm[e];
         d.stf_off = (int32_t)stf.size();
         d.load_off = (int32_t)load.size();
         d.shot = sl;
@@ -997,7 +1152,11 @@ int sem_batch_prepare(sem_batch *b, int path, int epb, int spg) {
         double ref[3];
         if ((rc = locate(m, s.recs[r].data(), &e, ref))) return fail(rc);
         RecDev d;
-        d.elem = m->inv_perm[e];
+        d.nodes_off = (int32_t)rec_nodes.size();
+        for (int p = 0; p < m->nd; ++p) {
+          int32_t gid = m->conn[e * m->nd + p];
+          rec_nodes.push_back((uint32_t)(g.tiled ? m->tiles.to_internal[gid] : gid));
+        }
         d.w_off = (int32_t)rec_w.size();
         d.shot = sl;
         d.stride = (int32_t)(s.recs.size() * m->c);
@@ -1023,14 +1182,20 @@ int sem_batch_prepare(sem_batch *b, int path, int epb, int spg) {
       if ((rc = hip_check(hipMemcpyAsync(g.d_stf, stf.data(), stf.size() * 4, hipMemcpyHostToDevice, b->stream), "upload"))) return fail(rc);
       if ((rc = hip_check(hipMemcpyAsync(g.d_load, load.data(), load.size() * 4, hipMemcpyHostToDevice, b->stream), "upload"))) return fail(rc);
     }
-    if (path == SEM_PATH_PLAIN || !m->affine)
+    if (path == SEM_PATH_PLAIN || (!m->affine && !g.tiled))
       if ((rc = ensure_pointwise(m))) return fail(rc);
     GroupDev h{};
     h.conn = m->d_conn;
+    if (g.tiled) {
+      h.tiles = m->d_tile_info;
+      h.tile_words = m->d_tile_words;
+      h.tgeo_e = m->d_tgeo_e;
+      h.tpar_e = m->d_tpar_e;
+    }
     h.geo_e = m->d_geo_e;
     h.par_e = m->d_par_e;
     h.geo_p = m->d_geo_p;
-    h.mi = m->d_mi;
+    h.mi = g.tiled ? m->d_mi_int : m->d_mi;
     h.u = g.fields;
     h.v = g.fields + len;
     h.ap = g.fields + 2 * len;
@@ -1063,6 +1228,9 @@ int sem_batch_prepare(sem_batch *b, int path, int epb, int spg) {
   if ((rc = dalloc(&b->d_seis, (size_t)std::max<int64_t>(seis, 1)))) return fail(rc);
   if ((rc = launch_fill_f32(std::max<int64_t>(seis, 1), b->d_seis, 0.f, b->stream))) return fail(rc);
   if ((rc = dalloc(&b->d_rec_w, rec_w.size()))) return fail(rc);
+  if ((rc = dalloc(&b->d_rec_nodes, rec_nodes.size()))) return fail(rc);
+  if (!rec_nodes.empty())
+    if ((rc = hip_check(hipMemcpyAsync(b->d_rec_nodes, rec_nodes.data(), rec_nodes.size() * 4, hipMemcpyHostToDevice, b->stream), "upload"))) return fail(rc);
   if (!rec_w.empty())
     if ((rc = hip_check(hipMemcpyAsync(b->d_rec_w, rec_w.data(), rec_w.size() * 4, hipMemcpyHostToDevice, b->stream), "upload"))) return fail(rc);
   for (auto &kv : rec_key) {
@@ -1213,7 +1381,10 @@ int sem_mesh_apply_stiffness(sem_mesh *m, int path, const float *u, float *ku) {
   int rc = sem_batch_add_shot(b, m, 1, 1.0, &shot);
   if (!rc) rc = sem_batch_prepare(b, path, path == SEM_PATH_FUSED ? default_epb(m->nd) : 0, 1);
   const size_t len = (size_t)m->c * m->G;
-  if (!rc) rc = hip_check(hipMemcpyAsync(b->h_groups[0].u, u, len * 4, hipMemcpyHostToDevice, b->stream), "upload");
+  std::vector<float> uin(u, u + len);
+  if (!rc && b->groups[0].tiled)
+    for (size_t i = 0; i < len; ++i) uin[field_pos(b->groups[0], i)] = u[i];
+  if (!rc) rc = hip_check(hipMemcpyAsync(b->h_groups[0].u, uin.data(), len * 4, hipMemcpyHostToDevice, b->stream), "upload");
   if (!rc) {
     if (path == SEM_PATH_FUSED) {
       rc = enqueue_fused_step(b, 0, 1);
@@ -1238,7 +1409,7 @@ int sem_mesh_apply_stiffness(sem_mesh *m, int path, const float *u, float *ku) {
   if (!rc) rc = hip_check(hipMemcpyAsync(acc.data(), b->h_groups[0].acc, len * 4, hipMemcpyDeviceToHost, b->stream), "copy");
   if (!rc) rc = hip_check(hipStreamSynchronize(b->stream), "sync");
   if (!rc)
-    for (size_t i = 0; i < len; ++i) ku[i] = -acc[i];
+    for (size_t i = 0; i < len; ++i) ku[i] = -acc[field_pos(b->groups[0], i)];
   sem_batch_destroy(b);
   return rc;
 }

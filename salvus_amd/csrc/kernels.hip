@@ -526,6 +526,15 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
   }
 }
 
+
+#include "tile_kernel.inc"
+
+__global__ void k_permute_f32(int64_t n, const int32_t *__restrict__ to, const float *__restrict__ src,
+                              float *__restrict__ dst) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[to[i]] = src[i];
+}
+
 // ------------------------------------------------------------------ plain path
 // One shot per launch; S = 1 layout.  gather -> stiffness -> colour scatter
 // -> update, each a separate kernel with element-local buffers in HBM.
@@ -630,10 +639,10 @@ __global__ void k_receivers(RecArgs A, int C) {
   const int64_t step = *G.step + A.koff;
   if (step >= G.nsteps) return;
   const RecDev R = A.recs[r];
-  const uint32_t *cn = G.conn + (int64_t)R.elem * A.nd;
+  const uint32_t *nodes = A.nodes + R.nodes_off;
   float s = 0.f;
   for (int p = 0; p < A.nd; ++p)
-    s = fmaf(A.weights[R.w_off + p], G.u[((int64_t)c * G.G + (cn[p] & kNodeMask)) * G.S + R.shot], s);
+    s = fmaf(A.weights[R.w_off + p], G.u[((int64_t)c * G.G + nodes[p]) * G.S + R.shot], s);
   A.out[R.out_off + step * R.stride + c] = s;
 }
 
@@ -766,6 +775,34 @@ int launch_fused(int dim, int order, int physics, bool affine, int nitems, int e
     switch (order) { SEM_ORDERS_2D(FU2) default: set_error("order not compiled"); return -1; }
   }
   return last_launch("fused step");
+}
+
+int launch_tile(int dim, int order, int physics, int nitems, int nd, int c, const FusedArgs &a,
+                hipStream_t st) {
+  if (nitems <= 0) return 0;
+  const int bt = block_threads(nd, a.sub_max);
+  if (bt > 512) { set_error("tile sub-round too large"); return -1; }
+  const size_t shm = tile_lds_bytes(dim, physics, nd, c, a.ntn_max, a.sub_max, a.epb);
+  if (shm > 160 * 1024) { set_error("tile does not fit in LDS"); return -1; }
+  dim3 grid(nitems), block(bt);
+#define TL(D, N) case N: \
+    if (physics == SEM_ACOUSTIC) hipLaunchKernelGGL((k_tile<D, N, 0>), grid, block, shm, st, a); \
+    else hipLaunchKernelGGL((k_tile<D, N, 1>), grid, block, shm, st, a); \
+    break;
+#define TL3(N) TL(3, N)
+#define TL2(N) TL(2, N)
+  if (dim == 3) {
+    switch (order) { SEM_ORDERS_3D(TL3) default: set_error("order not compiled"); return -1; }
+  } else {
+    switch (order) { SEM_ORDERS_2D(TL2) default: set_error("order not compiled"); return -1; }
+  }
+  return last_launch("tile step");
+}
+
+int launch_permute_f32(int64_t n, const int32_t *to, const float *src, float *dst, hipStream_t st) {
+  if (n <= 0) return 0;
+  k_permute_f32<<<nblocks(n, 256), 256, 0, st>>>(n, to, src, dst);
+  return last_launch("permute");
 }
 
 int launch_plain_gather(int64_t E, int nd, int C, int64_t G, const uint32_t *conn, const float *u,

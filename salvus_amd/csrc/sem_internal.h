@@ -92,11 +92,20 @@ struct SrcDev {
 };
 
 struct RecDev {
-  int32_t elem;      // sorted element position
+  int32_t nodes_off; // offset of the owning element's nd node ids (field numbering)
   int32_t w_off;     // offset of nd Lagrange weights
   int32_t shot;      // shot slot within the group
   int32_t stride;    // floats between consecutive steps (R_shot * c)
   int64_t out_off;   // offset of this receiver's step-0, component-0 sample
+};
+
+// One tile of a logically Cartesian mesh (see tiles.cpp).
+struct TileInfo {
+  int32_t ex, ey, ez;  // elements per axis in this tile
+  int32_t colour;      // lattice parity colour
+  int32_t elem_off;    // first element in tile order
+  int32_t pad;
+  int64_t node_off;    // first node slot
 };
 
 // One group = shots sharing a mesh, fields interleaved shot-fastest:
@@ -119,14 +128,33 @@ struct GroupDev {
   float dt, acl, dsp;
   int32_t quirks;
   int32_t pad;
+  // tile kernel (structured meshes): fields in tile-major node numbering
+  const TileInfo *tiles;
+  const uint32_t *tile_words;
+  const float *tgeo_e;    // affine factors in tile element order
+  const float *tpar_e;    // vertex material in tile element order
 };
 
-// Work item of a colour launch: EPB elements of one group.
+// Work item of a colour launch: EPB elements of one group (element kernel),
+// or one tile of one group (tile kernel: `first` = tile id).
 struct WorkItem {
   int32_t group;
-  int32_t first;  // first sorted element position
+  int32_t first;  // first sorted element position / tile id
   int32_t count;  // elements in this item
   int32_t pad;
+};
+
+struct TileSet {
+  bool valid = false;
+  int T[3] = {0, 0, 0};
+  int64_t ntiles = 0, total_slots = 0;
+  std::vector<TileInfo> info;
+  std::vector<uint32_t> words;       // slot -> internal node | flags
+  std::vector<int32_t> to_internal;  // DMPlex DOF -> internal node
+  std::vector<int32_t> elem_order;   // tile-order position -> element
+  std::vector<int32_t> elem_pos;     // element -> tile-order position
+  std::vector<int64_t> colour_start; // tiles by colour
+  std::vector<int32_t> by_colour;
 };
 
 // Kernel argument blocks (passed by value; recorded by graph capture).
@@ -137,6 +165,8 @@ struct FusedArgs {
   int32_t epb;
   int32_t koff;      // step offset within the captured block
   int32_t ku_only;   // 1: accumulate -K u into acc on every colour (no update)
+  int32_t ntn_max;   // tile kernel: largest tile node lattice in this launch
+  int32_t sub_max;   // tile kernel: largest same-parity element set of a tile
 };
 
 struct PlainArgs {
@@ -159,6 +189,7 @@ struct RecArgs {
   const GroupDev *groups;
   const int32_t *rec_group;  // group of each receiver
   const RecDev *recs;
+  const uint32_t *nodes;
   const float *weights;
   float *out;
   int32_t nrec;
@@ -189,6 +220,20 @@ inline size_t fused_lds_bytes(int dim, int nd, int c, int epb) {
 }
 int launch_fused(int dim, int order, int physics, bool affine, int nitems, int epb, int nd, int c,
                  const FusedArgs &a, hipStream_t st);
+// floats of per-element data a tile stages in LDS (factors + vertex material)
+inline int tile_elem_floats(int dim, int physics) {
+  const int ng = physics == SEM_ACOUSTIC ? dim * (dim + 1) / 2 : dim * dim + 1;
+  const int kp = physics == SEM_ACOUSTIC ? 1 : (dim == 2 ? 4 : 6);
+  return ng + (dim == 2 ? 4 : 8) * kp;
+}
+inline size_t tile_lds_bytes(int dim, int physics, int nd, int c, int ntn_max, int sub_max,
+                             int emax) {
+  return sizeof(float) * ((size_t)2 * c * ntn_max + ntn_max + (size_t)sub_max * c * dim * nd +
+                          (size_t)emax * tile_elem_floats(dim, physics));
+}
+int launch_tile(int dim, int order, int physics, int nitems, int nd, int c, const FusedArgs &a,
+                hipStream_t st);
+int launch_permute_f32(int64_t n, const int32_t *to, const float *src, float *dst, hipStream_t st);
 int launch_plain_gather(int64_t E, int nd, int C, int64_t G, const uint32_t *conn, const float *u,
                         float *uloc, hipStream_t st);
 int launch_plain_stiffness(int dim, int order, int physics, int64_t E, int nd, int c,
@@ -227,6 +272,11 @@ struct sem_mesh {
   float *d_mi = nullptr;
   double *d_vtx = nullptr;         // sorted E x nv x dim (FP64)
   double *d_vpar = nullptr;        // sorted E x nv x npar (FP64)
+  // tile path (logically Cartesian affine meshes)
+  sem::TileSet tiles;
+  sem::TileInfo *d_tile_info = nullptr;
+  uint32_t *d_tile_words = nullptr;
+  float *d_tgeo_e = nullptr, *d_tpar_e = nullptr, *d_mi_int = nullptr;
   size_t bytes = 0;
 };
 
@@ -236,3 +286,8 @@ struct Group;
 }  // namespace sem
 
 struct sem_batch;
+
+namespace sem {
+// Tiles a logically Cartesian mesh; 0 = tiled, 1 = not eligible.
+int build_tiles(sem_mesh *m, int Tx, int Ty, int Tz, TileSet &ts);
+}  // namespace sem

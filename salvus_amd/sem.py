@@ -159,11 +159,14 @@ class Mesh:
         self._h = _VP()
         _check(lib().sem_mesh_create(C.byref(self._h), device, dim, order, physics, self.E, self.V,
                                      _i(cells), _d(coords), _d(par), quirks))
-        info = np.zeros(8, dtype=np.int64)
+        info = np.zeros(10, dtype=np.int64)
         lib().sem_mesh_info(self._h, _l(info))
         self.G, self.ncolours, self.c, self.nd = (int(x) for x in info[1:5])
         self.affine = bool(info[5])
         self.device_bytes = int(info[6])
+        self.ntiles = int(info[8])
+        t = int(info[9])
+        self.tile_shape = (t // 10000, (t // 100) % 100, t % 100) if t else None
 
     def close(self):
         if getattr(self, "_h", None):

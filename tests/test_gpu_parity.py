@@ -85,6 +85,7 @@ def _run_both(P, path, quirks=po.DEFAULT_QUIRKS, init=None):
         prob.set_field("u", init)
     seis_ref = S.run(P["dt"], P["nsteps"])
     prob.takeTimeStep(P["nsteps"])
+    prob.mesh = M
     return (prob.seismograms(), seis_ref, prob.get_field("u"), S.get_field("u"), prob, S)
 
 
@@ -203,3 +204,32 @@ def test_moment_tensor_source_matches_oracle():
     ref = S.run(P["dt"], P["nsteps"])
     prob.takeTimeStep(P["nsteps"])
     assert pb.rel_l2(prob.seismograms(), ref) < TOL_RUN
+
+
+@pytest.mark.parametrize("dim,physics,nel", [(2, 0, 13), (2, 1, 9), (3, 0, 5), (3, 1, 6)])
+def test_tile_kernel_matches_element_kernel_and_oracle(dim, physics, nel, monkeypatch):
+    """Structured meshes take the tile kernel (partial tiles included); the
+    element-colour kernel (SEM_NO_TILES=1) and the oracle agree with it."""
+    P = pb.make(dim, physics, nel, order=4, seed=12, nsteps=150)
+    monkeypatch.setenv("SEM_NO_TILES", "0")
+    seis_t, ref, u_t, u_ref, prob, _ = _run_both(P, sem.PATH_FUSED)
+    assert prob.mesh.ntiles > 0
+    monkeypatch.setenv("SEM_NO_TILES", "1")
+    seis_e, _, u_e, _, prob_e, _ = _run_both(P, sem.PATH_FUSED)
+    assert prob_e.mesh.ntiles == 0
+    assert pb.rel_l2(seis_t, ref) < TOL_RUN and pb.rel_l2(u_t, u_ref) < TOL_RUN
+    assert pb.rel_l2(seis_t, seis_e) < TOL_RUN
+
+
+def test_distorted_mesh_uses_element_kernel():
+    P = pb.make(3, 0, 3, order=4, seed=13, distorted=True)
+    assert mesh_for(P).ntiles == 0
+
+
+@pytest.mark.parametrize("tile", ["2,2,2", "4,4,4", "3,1,2"])
+def test_tile_shapes(tile, monkeypatch):
+    monkeypatch.setenv("SEM_TILE", tile)
+    P = pb.make(3, 1, 5, order=3, seed=14, nsteps=100)
+    seis, ref, u, u_ref, prob, _ = _run_both(P, sem.PATH_FUSED)
+    assert prob.mesh.ntiles > 0
+    assert pb.rel_l2(seis, ref) < TOL_RUN and pb.rel_l2(u, u_ref) < TOL_RUN
