@@ -89,7 +89,7 @@ struct LaunchSet {
   bool affine;
   bool tile;
   int nitems;
-  int ntn_max, sub_max, emax;
+  int T[3];  // tile shape (tile kernel)
   WorkItem *d_items;
 };
 
@@ -141,7 +141,8 @@ static int set_device(int device) {
     return -1;
   }
   HIPCHK(hipSetDevice(device));
-  return init_device_tables(device);
+  SEM_TRY(init_device_tables(device));
+  return init_tile_tables(device);
 }
 
 static bool is_affine(int dim, const double *v) {
@@ -213,18 +214,28 @@ static int ensure_pointwise(sem_mesh *m) {
   return 0;
 }
 
-// Default tile shape per dimension/physics (overridable with SEM_TILE=x,y[,z]).
-static void default_tile(int dim, int physics, int *T) {
-  if (dim == 2) { T[0] = 8; T[1] = 8; T[2] = 1; }
-  else if (physics == SEM_ACOUSTIC) { T[0] = 4; T[1] = 4; T[2] = 2; }
-  else { T[0] = 4; T[1] = 2; T[2] = 2; }
+// Tile shape: the first compiled shape of the preference list for this
+// dimension/physics, or exactly SEM_TILE=x,y[,z] (no tiles if that shape has
+// no kernel for the mesh's order).  Returns false when no shape applies.
+static bool pick_tile(int dim, int order, int physics, int *T) {
   const char *env = getenv("SEM_TILE");
   if (env && *env) {
     int a = 0, b = 0, c = 1;
-    if (sscanf(env, "%d,%d,%d", &a, &b, &c) >= 2 && a > 0 && b > 0 && c > 0) {
-      T[0] = a; T[1] = b; T[2] = dim == 3 ? c : 1;
-    }
+    if (sscanf(env, "%d,%d,%d", &a, &b, &c) < 2 || a <= 0 || b <= 0 || c <= 0) return false;
+    T[0] = a; T[1] = b; T[2] = dim == 3 ? c : 1;
+    return tile_kernel_info(dim, order, physics, T).lds_bytes > 0;
   }
+  static const int pref2[][3] = {{16, 8, 1}, {8, 8, 1}};
+  static const int pref3a[][3] = {{4, 4, 2}, {4, 2, 2}, {2, 2, 2}};
+  static const int pref3e[][3] = {{2, 2, 2}, {4, 2, 2}};
+  const int(*list)[3] = dim == 2 ? pref2 : (physics == SEM_ACOUSTIC ? pref3a : pref3e);
+  const int cnt = dim == 2 ? 2 : (physics == SEM_ACOUSTIC ? 3 : 2);
+  for (int i = 0; i < cnt; ++i) {
+    T[0] = list[i][0]; T[1] = list[i][1]; T[2] = list[i][2];
+    const TileKernelInfo k = tile_kernel_info(dim, order, physics, T);
+    if (k.lds_bytes > 0 && k.lds_bytes <= 64 * 1024) return true;
+  }
+  return false;
 }
 
 static int upload_tile_words(sem_mesh *m) {
@@ -240,24 +251,15 @@ static int setup_tiles(sem_mesh *m) {
   const char *off = getenv("SEM_NO_TILES");
   if (!m->affine || (off && off[0] == '1')) return 0;
   int T[3];
-  default_tile(m->dim, m->physics, T);
-  // shrink the tile until its LDS footprint fits 64 KiB
-  for (;;) {
-    const int N = m->order;
-    int ntn = (T[0] * N + 1) * (T[1] * N + 1) * (m->dim == 3 ? T[2] * N + 1 : 1);
-    int sub = ((T[0] + 1) / 2) * ((T[1] + 1) / 2) * (m->dim == 3 ? (T[2] + 1) / 2 : 1);
-    sub = std::max(1, std::min(sub, 512 / m->nd));
-    if (tile_lds_bytes(m->dim, m->physics, m->nd, m->c, ntn, sub, T[0] * T[1] * T[2]) <= 64 * 1024 || (T[0] == 1 && T[1] == 1 && T[2] == 1)) break;
-    int a = (T[0] >= T[1] && T[0] >= T[2]) ? 0 : (T[1] >= T[2] ? 1 : 2);
-    T[a] = std::max(1, T[a] / 2);
-  }
+  if (!pick_tile(m->dim, m->order, m->physics, T)) return 0;
   if (build_tiles(m, T[0], T[1], T[2], m->tiles) != 0) {
     m->tiles = TileSet();
     return 0;  // not eligible: the element kernel is used
   }
   TileSet &ts = m->tiles;
   const int dim = m->dim, nv = m->nv, npar = ref_npar(dim, m->physics), kp = kernel_npar(dim, m->physics);
-  const int64_t E = m->E;
+  // padded tile element order (padding repeats element 0; never read)
+  const int64_t E = (int64_t)ts.elem_order.size();
   SEM_TRY(dalloc(&m->d_tile_info, ts.info.size(), &m->bytes));
   HIPCHK(hipMemcpyAsync(m->d_tile_info, ts.info.data(), ts.info.size() * sizeof(TileInfo), hipMemcpyHostToDevice, m->stream));
   SEM_TRY(upload_tile_words(m));
@@ -265,7 +267,7 @@ static int setup_tiles(sem_mesh *m) {
   std::vector<double> tv(E * nv * dim);
   std::vector<float> tp(E * nv * kp);
   for (int64_t i = 0; i < E; ++i) {
-    int64_t e = ts.elem_order[i];
+    const int64_t e = std::max<int32_t>(ts.elem_order[i], 0);
     std::copy_n(&m->vtx[e * nv * dim], nv * dim, &tv[i * nv * dim]);
     for (int k = 0; k < nv; ++k) {
       const double *src = &m->vpar[(e * nv + k) * npar];
@@ -548,6 +550,7 @@ int sem_mesh_set_dirichlet(sem_mesh *m, const uint8_t *mask) {
     std::vector<uint8_t> dint(m->G, 0);
     for (int64_t g = 0; g < m->G; ++g) dint[ts.to_internal[g]] = m->dirichlet[g];
     for (auto &w : ts.words) {
+      if (w == kSlotEmpty) continue;
       uint32_t node = w & kNodeMask;
       w = node | (w & flags) | (dint[node] ? kFlagDirichlet : 0u);
     }
@@ -865,12 +868,13 @@ static int default_epb(int nd) {
 static double time_shape(sem_batch *b, int epb, int reps);
 
 static int build_plan(sem_batch *b) {
-  // work items per (kernel key, colour); key = (tile?, dim, order, physics, affine)
-  std::map<std::tuple<int, int, int, int, int>, std::vector<int>> keys;
+  // work items per (kernel key, colour); key = (tile?, dim, order, physics, affine, tile shape)
+  std::map<std::tuple<int, int, int, int, int, int>, std::vector<int>> keys;
   for (int gi = 0; gi < (int)b->groups.size(); ++gi) {
     sem_mesh *m = b->groups[gi].mesh;
-    keys[std::make_tuple((int)b->groups[gi].tiled, m->dim, m->order, m->physics, (int)m->affine)]
-        .push_back(gi);
+    const bool t = b->groups[gi].tiled;
+    const int shape = t ? m->tiles.T[0] * 10000 + m->tiles.T[1] * 100 + m->tiles.T[2] : 0;
+    keys[std::make_tuple((int)t, m->dim, m->order, m->physics, (int)m->affine, shape)].push_back(gi);
   }
   for (auto &l : b->launches) dfree(l.d_items);
   b->launches.clear();
@@ -883,7 +887,6 @@ static int build_plan(sem_batch *b) {
     }
     for (int c = 0; c < maxcol; ++c) {
       std::vector<WorkItem> items;
-      int ntn_max = 0, sub_max = 0, emax = 0;
       for (int gi : kv.second) {
         sem_mesh *m = b->groups[gi].mesh;
         if (tile) {
@@ -899,11 +902,6 @@ static int build_plan(sem_batch *b) {
             w.pad = 0;
             items.push_back(w);
           }
-          const int N = m->order;
-          ntn_max = std::max(ntn_max, (ts.T[0] * N + 1) * (ts.T[1] * N + 1) * (m->dim == 3 ? ts.T[2] * N + 1 : 1));
-          int sub = ((ts.T[0] + 1) / 2) * ((ts.T[1] + 1) / 2) * (m->dim == 3 ? (ts.T[2] + 1) / 2 : 1);
-          sub_max = std::max(sub_max, std::max(1, std::min(sub, 512 / m->nd)));
-          emax = std::max(emax, ts.T[0] * ts.T[1] * (m->dim == 3 ? ts.T[2] : 1));
         } else {
           if (c >= m->ncol) continue;
           for (int64_t f = m->col_start[c]; f < m->col_start[c + 1]; f += b->epb) {
@@ -927,9 +925,10 @@ static int build_plan(sem_batch *b) {
       L.c = L.physics == SEM_ACOUSTIC ? 1 : L.dim;
       L.colour = c;
       L.nitems = (int)items.size();
-      L.ntn_max = ntn_max;
-      L.sub_max = sub_max;
-      L.emax = emax;
+      const int shape = std::get<5>(kv.first);
+      L.T[0] = shape / 10000;
+      L.T[1] = (shape / 100) % 100;
+      L.T[2] = shape % 100;
       SEM_TRY(dalloc(&L.d_items, items.size()));
       HIPCHK(hipMemcpyAsync(L.d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, b->stream));
       b->launches.push_back(L);
@@ -948,11 +947,8 @@ static int enqueue_fused_step(sem_batch *b, int koff, int ku_only) {
     a.epb = b->epb;
     a.koff = koff;
     a.ku_only = ku_only;
-    a.ntn_max = L.ntn_max;
-    a.sub_max = L.sub_max;
-    if (L.tile) a.epb = L.emax;
     auto go = [&]() {
-      return L.tile ? launch_tile(L.dim, L.order, L.physics, L.nitems, L.nd, L.c, a, b->stream)
+      return L.tile ? launch_tile(L.dim, L.order, L.physics, L.T, L.nitems, a, b->stream)
                     : launch_fused(L.dim, L.order, L.physics, L.affine, L.nitems, b->epb, L.nd, L.c, a, b->stream);
     };
     if (b->timing && !ku_only) {
@@ -1099,7 +1095,22 @@ int sem_batch_prepare(sem_batch *b, int path, int epb, int spg) {
   for (int gi = 0; gi < (int)b->groups.size(); ++gi) {
     Group &g = b->groups[gi];
     sem_mesh *m = g.mesh;
-    g.tiled = path == SEM_PATH_FUSED && m->tiles.valid;
+    // tile kernel: 32-bit field offsets and at most kTileMaxSrc sources per tile
+    g.tiled = path == SEM_PATH_FUSED && m->tiles.valid &&
+              (uint64_t)m->c * (uint64_t)m->G * (uint64_t)g.S < (1ull << 32);
+    if (g.tiled) {
+      std::map<int64_t, int> per_tile;
+      for (int sl = 0; sl < g.S && g.tiled; ++sl)
+        for (auto &sp : b->shots[g.shots[sl]].srcs) {
+          int64_t e;
+          std::vector<double> ld;
+          if ((rc = source_load(m, sp, &e, ld))) return fail(rc);
+          if (++per_tile[m->tiles.elem_pos[e] / m->tiles.elems_per_tile] > kTileMaxSrc) {
+            g.tiled = false;
+            break;
+          }
+        }
+    }
     const size_t len = (size_t)m->c * m->G * g.S;
     if ((rc = dalloc(&g.fields, 6 * len))) return fail(rc);
     std::vector<float> buf(len);

@@ -526,9 +526,6 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
   }
 }
 
-
-#include "tile_kernel.inc"
-
 __global__ void k_permute_f32(int64_t n, const int32_t *__restrict__ to, const float *__restrict__ src,
                               float *__restrict__ dst) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -775,28 +772,6 @@ int launch_fused(int dim, int order, int physics, bool affine, int nitems, int e
     switch (order) { SEM_ORDERS_2D(FU2) default: set_error("order not compiled"); return -1; }
   }
   return last_launch("fused step");
-}
-
-int launch_tile(int dim, int order, int physics, int nitems, int nd, int c, const FusedArgs &a,
-                hipStream_t st) {
-  if (nitems <= 0) return 0;
-  const int bt = block_threads(nd, a.sub_max);
-  if (bt > 512) { set_error("tile sub-round too large"); return -1; }
-  const size_t shm = tile_lds_bytes(dim, physics, nd, c, a.ntn_max, a.sub_max, a.epb);
-  if (shm > 160 * 1024) { set_error("tile does not fit in LDS"); return -1; }
-  dim3 grid(nitems), block(bt);
-#define TL(D, N) case N: \
-    if (physics == SEM_ACOUSTIC) hipLaunchKernelGGL((k_tile<D, N, 0>), grid, block, shm, st, a); \
-    else hipLaunchKernelGGL((k_tile<D, N, 1>), grid, block, shm, st, a); \
-    break;
-#define TL3(N) TL(3, N)
-#define TL2(N) TL(2, N)
-  if (dim == 3) {
-    switch (order) { SEM_ORDERS_3D(TL3) default: set_error("order not compiled"); return -1; }
-  } else {
-    switch (order) { SEM_ORDERS_2D(TL2) default: set_error("order not compiled"); return -1; }
-  }
-  return last_launch("tile step");
 }
 
 int launch_permute_f32(int64_t n, const int32_t *to, const float *src, float *dst, hipStream_t st) {

@@ -22,6 +22,11 @@ constexpr uint32_t kNodeMask = (1u << 29) - 1;
 constexpr uint32_t kFlagDirichlet = 1u << 29;  // zero the stiffness row (Q14)
 constexpr uint32_t kFlagFirst = 1u << 30;      // first colour to touch the node: store
 constexpr uint32_t kFlagLast = 1u << 31;       // last colour: finalise (mass + Newmark)
+// Tile lattice slot outside the mesh (partial tile at the mesh edge).
+constexpr uint32_t kSlotEmpty = 0xffffffffu;
+// Sources (over all shots of a group) one tile may hold; groups with more
+// in some tile take the element kernel.
+constexpr int kTileMaxSrc = 64;
 
 // ---------------------------------------------------------------- errors
 void set_error(const std::string &msg);
@@ -144,14 +149,19 @@ struct WorkItem {
   int32_t pad;
 };
 
+// Every tile has the full Tx x Ty (x Tz) element box and its whole node
+// lattice: elements and slots beyond a partial tile at the mesh edge are
+// padding (element order -1, slot word kSlotEmpty).
 struct TileSet {
   bool valid = false;
   int T[3] = {0, 0, 0};
   int64_t ntiles = 0, total_slots = 0;
+  int64_t slots_per_tile = 0;        // (Tx N + 1)(Ty N + 1)(Tz N + 1)
+  int elems_per_tile = 0;            // Tx Ty Tz
   std::vector<TileInfo> info;
-  std::vector<uint32_t> words;       // slot -> internal node | flags
+  std::vector<uint32_t> words;       // slot -> internal node | flags, or kSlotEmpty
   std::vector<int32_t> to_internal;  // DMPlex DOF -> internal node
-  std::vector<int32_t> elem_order;   // tile-order position -> element
+  std::vector<int32_t> elem_order;   // tile-order position -> element (-1: padding)
   std::vector<int32_t> elem_pos;     // element -> tile-order position
   std::vector<int64_t> colour_start; // tiles by colour
   std::vector<int32_t> by_colour;
@@ -165,8 +175,6 @@ struct FusedArgs {
   int32_t epb;
   int32_t koff;      // step offset within the captured block
   int32_t ku_only;   // 1: accumulate -K u into acc on every colour (no update)
-  int32_t ntn_max;   // tile kernel: largest tile node lattice in this launch
-  int32_t sub_max;   // tile kernel: largest same-parity element set of a tile
 };
 
 struct PlainArgs {
@@ -220,19 +228,6 @@ inline size_t fused_lds_bytes(int dim, int nd, int c, int epb) {
 }
 int launch_fused(int dim, int order, int physics, bool affine, int nitems, int epb, int nd, int c,
                  const FusedArgs &a, hipStream_t st);
-// floats of per-element data a tile stages in LDS (factors + vertex material)
-inline int tile_elem_floats(int dim, int physics) {
-  const int ng = physics == SEM_ACOUSTIC ? dim * (dim + 1) / 2 : dim * dim + 1;
-  const int kp = physics == SEM_ACOUSTIC ? 1 : (dim == 2 ? 4 : 6);
-  return ng + (dim == 2 ? 4 : 8) * kp;
-}
-inline size_t tile_lds_bytes(int dim, int physics, int nd, int c, int ntn_max, int sub_max,
-                             int emax) {
-  return sizeof(float) * ((size_t)2 * c * ntn_max + ntn_max + (size_t)sub_max * c * dim * nd +
-                          (size_t)emax * tile_elem_floats(dim, physics));
-}
-int launch_tile(int dim, int order, int physics, int nitems, int nd, int c, const FusedArgs &a,
-                hipStream_t st);
 int launch_permute_f32(int64_t n, const int32_t *to, const float *src, float *dst, hipStream_t st);
 int launch_plain_gather(int64_t E, int nd, int C, int64_t G, const uint32_t *conn, const float *u,
                         float *uloc, hipStream_t st);
@@ -245,6 +240,16 @@ int launch_plain_update(int64_t G, int C, int quirks, float dt, float acl, float
                         float *ku, float *kv, hipStream_t st);
 int launch_receivers(const RecArgs &a, int C, hipStream_t st);
 int launch_advance(int ngroups, const GroupDev *groups, int64_t *steps, int64_t k, hipStream_t st);
+
+// ---------------------------------------------------------------- tile kernel (tile_kernels.hip)
+int init_tile_tables(int device);
+struct TileKernelInfo {
+  size_t lds_bytes;  // 0: no kernel compiled for this order and tile shape
+  int threads;
+};
+TileKernelInfo tile_kernel_info(int dim, int order, int physics, const int *T);
+int launch_tile(int dim, int order, int physics, const int *T, int nitems, const FusedArgs &a,
+                hipStream_t st);
 
 }  // namespace sem
 

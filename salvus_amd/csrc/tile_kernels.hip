@@ -1,0 +1,586 @@
+// Column-mapped tile kernel of the MI355X SEM path (gfx950).
+//
+// Structured (logically Cartesian, affine) meshes are cut into tiles of
+// TX x TY (x TZ) elements whose node lattice is staged in LDS; one workgroup
+// takes one tile, one launch takes the tiles of one lattice-parity colour.
+// One thread owns one node column of one element: the column runs along the
+// last reference axis (eta in 2-D, zeta in 3-D), so the derivative along it
+// is a register-only product with wave-uniform D entries, and only the
+// lateral derivatives read the LDS lattice.  Per shot:
+//   gather  u of every lattice slot -> LDS (loads batched per thread);
+//   column  reference gradient and J^-T flux per node; lateral fluxes go to
+//           LDS, the column flux stays in registers; the test-and-integrate
+//           result (-Ku, zero on Dirichlet rows, plus source loads) is
+//           stored element-locally in LDS;
+//   faces   element copies of shared nodes are summed in fixed passes along
+//           x, y (and z), after which the element of highest lattice
+//           position containing a node holds the tile's total for it;
+//   finish  every lattice slot is stored / accumulated in HBM, or -- on the
+//           last tile colour touching it -- finished: a = mi (acc + f);
+//           Kahan v += dt/2 (a + a_); Kahan u += dt v + dt^2/2 a; a_ = a
+//           (Order2Newmark.cpp:68-115).
+// The operator is the one of TensorQuad.cpp:176-210/:350-397 and
+// Hexahedra.cpp:579-754 (sum-factorised gradient, J^-T contraction,
+// test-and-integrate) with Scalar.cpp:42-71, Elastic2D.cpp:53-82 and
+// Elastic3D.cpp:33-54/:70-118 material laws.  All sums are in a fixed order,
+// so results are reproducible bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "sem_internal.h"
+
+namespace sem {
+
+namespace {
+
+constexpr int kTileMaxN = 8;
+
+struct TileTables {
+  float D[(kTileMaxN + 1) * (kTileMaxN + 1)];  // D[i*n+m] = l'_m(xi_i)
+  float w[kTileMaxN + 1];
+  float xi[kTileMaxN + 1];
+};
+__constant__ TileTables t_tab[kTileMaxN + 1];
+
+template <int DIM, int N, int PHYS, int TX, int TY, int TZ>
+struct TS {
+  static constexpr int n = N + 1;
+  static constexpr int ND = DIM == 2 ? n * n : n * n * n;  // nodes per element
+  static constexpr int NCOL = ND / n;                       // columns per element
+  static constexpr int C = PHYS == SEM_ACOUSTIC ? 1 : DIM;
+  static constexpr int NL = DIM - 1;                        // lateral axes
+  static constexpr int LX = TX * N + 1, LY = TY * N + 1, LZ = DIM == 3 ? TZ * N + 1 : 1;
+  static constexpr int LXY = LX * LY, NTL = LXY * LZ;       // lattice slots
+  static constexpr int CS = DIM == 3 ? LXY : LX;            // lattice stride of the column axis
+  static constexpr int ES = DIM == 3 ? n * n : n;           // element-local stride of the column axis
+  static constexpr int E = TX * TY * (DIM == 3 ? TZ : 1);
+  static constexpr int EC = E < 512 / NCOL ? E : 512 / NCOL;  // elements per column chunk
+  static constexpr int NCH = (E + EC - 1) / EC;
+  static constexpr int BT = ((EC * NCOL + 63) / 64) * 64;   // threads
+  static constexpr int NIT = (NTL + BT - 1) / BT;           // lattice slots per thread
+  static constexpr int NG = PHYS == SEM_ACOUSTIC ? DIM * (DIM + 1) / 2 : DIM * DIM + 1;
+  static constexpr int KP = PHYS == SEM_ACOUSTIC ? 1 : (DIM == 2 ? 4 : 6);
+  static constexpr int NV = DIM == 2 ? 4 : 8;
+  static constexpr int NED = NG + NV * KP;                  // floats of element data
+  // LDS layout (floats): u lattice | words | lateral fluxes of one chunk |
+  // element-local results | element data | tile source list (4 ints each)
+  static constexpr int O_SW = C * NTL;
+  static constexpr int O_SF = O_SW + NTL;
+  static constexpr int O_SK = O_SF + EC * C * NL * ND;
+  static constexpr int O_SE = O_SK + E * C * ND;
+  static constexpr int O_SS = O_SE + E * NED;
+  static constexpr int LDS = O_SS + 4 * kTileMaxSrc + 1;
+};
+
+__device__ __forceinline__ void kahan(float &s, float &c, float x) {
+  const float y = x - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+// P1 vertex weight at (a, b[, t]) with the reference's vertex order
+// (QuadP1.h:98-108, HexP1.cpp:242-298).  In 2-D the column coordinate t is eta.
+template <int DIM>
+__device__ __forceinline__ float p1w(int v, float a, float b, float t) {
+  if (DIM == 2) {
+    const float sa = (v == 1 || v == 2) ? 1.f : -1.f;
+    const float st = (v >= 2) ? 1.f : -1.f;
+    return 0.25f * (1.f + sa * a) * (1.f + st * t);
+  }
+  constexpr signed char hs[8][3] = {{-1, -1, -1}, {-1, 1, -1}, {1, 1, -1}, {1, -1, -1},
+                                    {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+  return 0.125f * (1.f + hs[v][0] * a) * (1.f + hs[v][1] * b) * (1.f + hs[v][2] * t);
+}
+
+template <int DIM, int N, int PHYS, int TX, int TY, int TZ>
+__global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT)) k_tile(FusedArgs A) {
+  using T = TS<DIM, N, PHYS, TX, TY, TZ>;
+  constexpr int n = T::n, C = T::C, ND = T::ND, NL = T::NL;
+  extern __shared__ float lds[];
+  float *su = lds;
+  uint32_t *sw = reinterpret_cast<uint32_t *>(lds + T::O_SW);
+  float *sF = lds + T::O_SF;
+  float *sK = lds + T::O_SK;
+  float *sE = lds + T::O_SE;
+  int *sS = reinterpret_cast<int *>(lds + T::O_SS);  // {element, shot, stf_off, load_off} x k, count
+
+  const int bid = blockIdx.x, nb = gridDim.x;
+  int item = bid;
+  if (nb >= 16 && (nb & 7) == 0) item = (bid & 7) * (nb >> 3) + (bid >> 3);
+  if (item >= A.nitems) return;
+  const WorkItem W = A.items[item];
+  const GroupDev &G = A.groups[W.group];
+  const int64_t step = *G.step + A.koff;
+  if (step >= G.nsteps) return;  // uniform across the block
+  const TileInfo ti = G.tiles[W.first];
+  const int tid = threadIdx.x;
+  const TileTables &TT = t_tab[N];
+
+  // ---- once per tile: node words, element data, sources inside the tile
+  {
+    const uint32_t *__restrict__ words = G.tile_words + ti.node_off;
+    uint32_t wv[T::NIT];
+#pragma unroll
+    for (int r = 0; r < T::NIT; ++r) {
+      const int l = tid + r * T::BT;
+      wv[r] = l < T::NTL ? words[l] : kSlotEmpty;
+    }
+#pragma unroll
+    for (int r = 0; r < T::NIT; ++r) {
+      const int l = tid + r * T::BT;
+      if (l < T::NTL) sw[l] = wv[r];
+    }
+    const float *__restrict__ tg = G.tgeo_e + (int64_t)ti.elem_off * T::NG;
+    const float *__restrict__ tp = G.tpar_e + (int64_t)ti.elem_off * (T::NV * T::KP);
+    for (int i = tid; i < T::E * T::NED; i += T::BT) {
+      const int e = i / T::NED, k = i - e * T::NED;
+      sE[i] = k < T::NG ? tg[e * T::NG + k] : tp[e * (T::NV * T::KP) + (k - T::NG)];
+    }
+    if (tid < 64) {  // ordered compaction by wave 0 (ballot + prefix count)
+      int cnt = 0;
+      for (int j0 = 0; j0 < G.nsrc; j0 += 64) {
+        const int j = j0 + tid;
+        SrcDev sd{0, 0, 0, 0};
+        bool in = false;
+        if (j < G.nsrc) {
+          sd = G.src[j];
+          in = sd.elem >= ti.elem_off && sd.elem < ti.elem_off + T::E;
+        }
+        const unsigned long long mask = __ballot(in);
+        if (in) {
+          const int pos = cnt + __popcll(mask & ((1ull << tid) - 1ull));
+          if (pos < kTileMaxSrc) {
+            sS[4 * pos + 0] = sd.elem - ti.elem_off;
+            sS[4 * pos + 1] = sd.shot;
+            sS[4 * pos + 2] = sd.stf_off;
+            sS[4 * pos + 3] = sd.load_off;
+          }
+        }
+        cnt += __popcll(mask);
+      }
+      if (tid == 0) sS[4 * kTileMaxSrc] = cnt < kTileMaxSrc ? cnt : kTileMaxSrc;
+    }
+  }
+
+  // ---- the column this thread owns in every chunk
+  const int ec = tid / T::NCOL;
+  const int col = tid - ec * T::NCOL;
+  const int ci = col % n;
+  const int cj = DIM == 3 ? col / n : 0;
+  const bool lane = ec < T::EC;
+  float Dx[n], Dy[n], Dcx[n], Dcy[n];
+#pragma unroll
+  for (int m = 0; m < n; ++m) {
+    Dx[m] = TT.D[ci * n + m];
+    Dcx[m] = TT.D[m * n + ci];
+    Dy[m] = DIM == 3 ? TT.D[cj * n + m] : 0.f;
+    Dcy[m] = DIM == 3 ? TT.D[m * n + cj] : 0.f;
+  }
+  const float wl = DIM == 3 ? TT.w[ci] * TT.w[cj] : TT.w[ci];
+  const float xa = TT.xi[ci], xb = DIM == 3 ? TT.xi[cj] : 0.f;
+  const int pl = ci + (DIM == 3 ? n * cj : 0);  // element-local index of the column's node 0
+
+  float *__restrict__ gu = G.u;
+  float *__restrict__ gv = G.v;
+  float *__restrict__ gap = G.ap;
+  float *__restrict__ gku = G.ku;
+  float *__restrict__ gkv = G.kv;
+  float *__restrict__ gacc = G.acc;
+  const float *__restrict__ gmi = G.mi;
+  const uint32_t GG = (uint32_t)G.G, Sg = (uint32_t)G.S;
+  const bool q3 = (C == 3) && (G.quirks & SEM_QUIRK_NEWMARK3D);
+
+  for (uint32_t s = 0; s < Sg; ++s) {
+    __syncthreads();  // tile data staged / previous shot done with su and sK
+    // ---- gather
+    {
+      float uv[T::NIT][C];
+      bool ok[T::NIT];
+#pragma unroll
+      for (int r = 0; r < T::NIT; ++r) {
+        const int l = tid + r * T::BT;
+        const uint32_t w = l < T::NTL ? sw[l] : kSlotEmpty;
+        ok[r] = w != kSlotEmpty;
+        const uint32_t g = ok[r] ? (w & kNodeMask) : 0u;
+#pragma unroll
+        for (int c = 0; c < C; ++c) uv[r][c] = ok[r] ? gu[(c * GG + g) * Sg + s] : 0.f;
+      }
+#pragma unroll
+      for (int r = 0; r < T::NIT; ++r) {
+        const int l = tid + r * T::BT;
+        if (ok[r]) {
+#pragma unroll
+          for (int c = 0; c < C; ++c) su[c * T::NTL + l] = uv[r][c];
+        }
+      }
+    }
+    __syncthreads();
+    const int nts = sS[4 * kTileMaxSrc];
+
+    // ---- columns, one chunk of elements at a time
+#pragma unroll 1
+    for (int ch = 0; ch < T::NCH; ++ch) {
+      if (ch > 0) __syncthreads();  // previous chunk has read its lateral fluxes
+      const int e = ch * T::EC + ec;
+      bool act = false;
+      int ebase = 0;
+      if (lane && e < T::E) {
+        const int ea = e % TX, eb = (e / TX) % TY, ez = DIM == 3 ? e / (TX * TY) : 0;
+        act = ea < ti.ex && eb < ti.ey && (DIM == 2 || ez < ti.ez);
+        ebase = ea * N + T::LX * (eb * N) + (DIM == 3 ? T::LXY * (ez * N) : 0);
+      }
+      const int cb = ebase + ci + (DIM == 3 ? T::LX * cj : 0);  // lattice slot of node 0
+      float *myF = sF + ec * (C * NL * ND);
+      float Fz[C][n];
+      if (act) {
+        const float *ed = sE + e * T::NED;
+        float uc[C][n];
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+          for (int k = 0; k < n; ++k) uc[c][k] = su[c * T::NTL + cb + k * T::CS];
+        // vertex material is trilinear: linear along the column
+        float mA[T::KP], mB[T::KP];
+        {
+          float w0[T::NV], w1[T::NV];
+#pragma unroll
+          for (int v = 0; v < T::NV; ++v) {
+            w0[v] = p1w<DIM>(v, xa, xb, 0.f);
+            w1[v] = p1w<DIM>(v, xa, xb, 1.f);
+          }
+#pragma unroll
+          for (int q = 0; q < T::KP; ++q) {
+            float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+            for (int v = 0; v < T::NV; ++v) {
+              a0 = fmaf(w0[v], ed[T::NG + v * T::KP + q], a0);
+              a1 = fmaf(w1[v], ed[T::NG + v * T::KP + q], a1);
+            }
+            mA[q] = a0;
+            mB[q] = a1 - a0;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+          const float tk = TT.xi[k], wk = TT.w[k];
+          float g[C][DIM];
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            const float *rx = su + c * T::NTL + (cb - ci) + k * T::CS;
+            float s0 = 0.f;
+#pragma unroll
+            for (int m = 0; m < n; ++m) s0 = fmaf(Dx[m], rx[m], s0);
+            g[c][0] = s0;
+            if (DIM == 3) {
+              const float *ry = su + c * T::NTL + (cb - T::LX * cj) + k * T::CS;
+              float s1 = 0.f;
+#pragma unroll
+              for (int m = 0; m < n; ++m) s1 = fmaf(Dy[m], ry[m * T::LX], s1);
+              g[c][NL > 1 ? 1 : 0] = s1;
+            }
+            float sz = 0.f;
+#pragma unroll
+            for (int m = 0; m < n; ++m) sz = fmaf(TT.D[k * n + m], uc[c][m], sz);
+            g[c][DIM - 1] = sz;
+          }
+          float F[C][DIM];
+          if constexpr (PHYS == SEM_ACOUSTIC) {
+            const float vp = fmaf(mB[0], tk, mA[0]);
+            const float sc = wl * wk * vp * vp;  // Scalar.cpp:45
+            const float *Gs = ed;
+            if constexpr (DIM == 2) {
+              F[0][0] = sc * (Gs[0] * g[0][0] + Gs[1] * g[0][1]);
+              F[0][1] = sc * (Gs[1] * g[0][0] + Gs[2] * g[0][1]);
+            } else {
+              F[0][0] = sc * (Gs[0] * g[0][0] + Gs[1] * g[0][1] + Gs[2] * g[0][DIM - 1]);
+              F[0][NL > 1 ? 1 : 0] = sc * (Gs[1] * g[0][0] + Gs[3] * g[0][1] + Gs[4] * g[0][DIM - 1]);
+              F[0][DIM - 1] = sc * (Gs[2] * g[0][0] + Gs[4] * g[0][1] + Gs[5] * g[0][DIM - 1]);
+            }
+          } else {
+            const float *Ji = ed;  // Jinv[a*DIM+b]
+            const float wdet = wl * wk * ed[DIM * DIM];
+            float cm[9];
+            if constexpr (DIM == 2) {  // C11, C13, C33, C55
+#pragma unroll
+              for (int q = 0; q < 4; ++q) cm[q] = fmaf(mB[q], tk, mA[q]);
+            } else {  // Elastic3D.cpp:42-52
+              float q6[6];
+#pragma unroll
+              for (int q = 0; q < 6; ++q) q6[q] = fmaf(mB[q], tk, mA[q]);
+              const float c11 = q6[0] * q6[2] * q6[2], c33 = q6[0] * q6[1] * q6[1];
+              const float c44 = q6[0] * q6[3] * q6[3], c66 = q6[0] * q6[4] * q6[4];
+              const float c13 = q6[5] * (c11 - 2.f * c44);
+              cm[0] = c11; cm[1] = c11 - 2.f * c66; cm[2] = c13; cm[3] = c11; cm[4] = c13;
+              cm[5] = c33; cm[6] = c44; cm[7] = c44; cm[8] = c66;
+            }
+            float gx[DIM][DIM];
+#pragma unroll
+            for (int c = 0; c < DIM; ++c)
+#pragma unroll
+              for (int a = 0; a < DIM; ++a) {
+                float v = 0.f;
+#pragma unroll
+                for (int b = 0; b < DIM; ++b) v = fmaf(Ji[a * DIM + b], g[c < C ? c : 0][b], v);
+                gx[c][a] = v;
+              }
+            float sig[DIM][DIM];
+            if constexpr (DIM == 2) {  // Elastic2D.cpp:53-82
+              const float exx = gx[0][0], eyy = gx[1][1], sh = gx[0][1] + gx[1][0];
+              sig[0][0] = cm[0] * exx + cm[1] * eyy;
+              sig[1][1] = cm[1] * exx + cm[2] * eyy;
+              sig[0][1] = sig[1][0] = cm[3] * sh;
+            } else {  // Elastic3D.cpp:76-83, :106-118
+              constexpr int Y = DIM > 2 ? 2 : 1;
+              const float e0 = gx[0][0], e1 = gx[1][1], e2 = gx[Y][Y];
+              const float e3 = gx[1][Y] + gx[Y][1], e4 = gx[0][Y] + gx[Y][0], e5 = gx[0][1] + gx[1][0];
+              sig[0][0] = cm[0] * e0 + cm[1] * e1 + cm[2] * e2;
+              sig[1][1] = cm[1] * e0 + cm[3] * e1 + cm[4] * e2;
+              sig[Y][Y] = cm[2] * e0 + cm[4] * e1 + cm[5] * e2;
+              sig[1][Y] = sig[Y][1] = cm[6] * e3;
+              sig[0][Y] = sig[Y][0] = cm[7] * e4;
+              sig[0][1] = sig[1][0] = cm[8] * e5;
+            }
+#pragma unroll
+            for (int c = 0; c < DIM; ++c)
+#pragma unroll
+              for (int a = 0; a < DIM; ++a) {
+                float v = 0.f;
+#pragma unroll
+                for (int b = 0; b < DIM; ++b) v = fmaf(Ji[b * DIM + a], sig[c][b], v);
+                F[c < C ? c : 0][a] = wdet * v;
+              }
+          }
+          const int p = pl + T::ES * k;
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+#pragma unroll
+            for (int a = 0; a < NL; ++a) myF[(c * NL + a) * ND + p] = F[c][a];
+            Fz[c][k] = F[c][DIM - 1];
+          }
+        }
+      }
+      __syncthreads();
+      if (act) {
+        float f[C][n];
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+          const int p = pl + T::ES * k;
+          const bool dir = (sw[cb + k * T::CS] & kFlagDirichlet) != 0;
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            const float *Fx = myF + (c * NL) * ND + (p - ci);  // F_x along the x line
+            float ku = 0.f;
+#pragma unroll
+            for (int m = 0; m < n; ++m) ku = fmaf(Dcx[m], Fx[m], ku);
+            if (DIM == 3) {
+              const float *Fy = myF + (c * NL + (NL > 1 ? 1 : 0)) * ND + (p - n * cj);
+#pragma unroll
+              for (int m = 0; m < n; ++m) ku = fmaf(Dcy[m], Fy[m * n], ku);
+            }
+#pragma unroll
+            for (int m = 0; m < n; ++m) ku = fmaf(TT.D[m * n + k], Fz[c][m], ku);
+            f[c][k] = dir ? 0.f : -ku;
+          }
+        }
+        for (int j = 0; j < nts; ++j) {
+          if (sS[4 * j + 0] != e || sS[4 * j + 1] != (int)s) continue;
+          const float R = G.stf[sS[4 * j + 2] + step];
+          const float *ld = G.load + sS[4 * j + 3];
+#pragma unroll
+          for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int k = 0; k < n; ++k) f[c][k] = fmaf(ld[c * ND + pl + T::ES * k], R, f[c][k]);
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+          for (int k = 0; k < n; ++k) sK[(e * C + c) * ND + pl + T::ES * k] = f[c][k];
+      }
+    }
+    __syncthreads();
+
+    // ---- shared faces: x, then y, then z (fixed order)
+#pragma unroll
+    for (int ax = 0; ax < DIM; ++ax) {
+      constexpr int NF = ND / n;
+      for (int i = tid; i < T::E * NF; i += T::BT) {
+        const int e = i / NF, q = i - e * NF;
+        const int ea = e % TX, eb = (e / TX) % TY, ez = DIM == 3 ? e / (TX * TY) : 0;
+        const bool valid = ea < ti.ex && eb < ti.ey && (DIM == 2 || ez < ti.ez);
+        int lo, p, pn;
+        if (ax == 0) {         // node (0, q) gains the x-neighbour's (N, q)
+          lo = ea > 0 ? e - 1 : -1;
+          p = q * n;
+          pn = p + N;
+        } else if (ax == 1) {  // (i, 0, k) gains (i, N, k)
+          lo = eb > 0 ? e - TX : -1;
+          p = DIM == 3 ? (q % n) + n * n * (q / n) : q;
+          pn = p + n * N;
+        } else {               // (i, j, 0) gains (i, j, N)
+          lo = ez > 0 ? e - TX * TY : -1;
+          p = q;
+          pn = q + n * n * N;
+        }
+        if (valid && lo >= 0) {
+#pragma unroll
+          for (int c = 0; c < C; ++c) sK[(e * C + c) * ND + p] += sK[(lo * C + c) * ND + pn];
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- finish every lattice slot
+    constexpr int KB = C == 1 ? 4 : 2;
+#pragma unroll 1
+    for (int r0 = 0; r0 < T::NIT; r0 += KB) {
+      uint32_t wk[KB];
+      float tot[KB][C], prev[KB][C], vv[KB][C], kvv[KB][C], kuu[KB][C], app[KB][C], mi[KB];
+#pragma unroll
+      for (int r = 0; r < KB; ++r) {
+        const int l = tid + (r0 + r) * T::BT;
+        wk[r] = (r0 + r < T::NIT && l < T::NTL) ? sw[l] : kSlotEmpty;
+        const bool valid = wk[r] != kSlotEmpty;
+        int e = 0, p = 0;
+        if (valid) {
+          const int x = l % T::LX, y = (l / T::LX) % T::LY, z = DIM == 3 ? l / T::LXY : 0;
+          const int ea = min(x / N, ti.ex - 1), eb = min(y / N, ti.ey - 1);
+          const int ez = DIM == 3 ? min(z / N, ti.ez - 1) : 0;
+          e = ea + TX * (eb + TY * ez);
+          p = (x - ea * N) + n * ((y - eb * N) + n * (z - ez * N));
+        }
+        const uint32_t g = valid ? (wk[r] & kNodeMask) : 0u;
+        const bool first = (wk[r] & kFlagFirst) != 0;
+        const bool fin = valid && (wk[r] & kFlagLast) && !A.ku_only;
+        mi[r] = fin ? gmi[g] : 0.f;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const uint32_t idx = (c * GG + g) * Sg + s;
+          tot[r][c] = valid ? sK[(e * C + c) * ND + p] : 0.f;
+          prev[r][c] = (valid && !first) ? gacc[idx] : 0.f;
+          app[r][c] = fin ? gap[idx] : 0.f;
+          vv[r][c] = fin ? gv[idx] : 0.f;
+          kvv[r][c] = fin ? gkv[idx] : 0.f;
+          kuu[r][c] = fin ? gku[idx] : 0.f;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < KB; ++r) {
+        if (wk[r] == kSlotEmpty) continue;
+        const int l = tid + (r0 + r) * T::BT;
+        const uint32_t g = wk[r] & kNodeMask;
+        const bool first = (wk[r] & kFlagFirst) != 0;
+        const bool fin = (wk[r] & kFlagLast) && !A.ku_only;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const uint32_t idx = (c * GG + g) * Sg + s;
+          const float t = first ? tot[r][c] : prev[r][c] + tot[r][c];
+          if (!fin) {
+            gacc[idx] = t;
+            continue;
+          }
+          if (q3 && c == 2) continue;
+          const float a = mi[r] * t;
+          float ap = app[r][c], v = vv[r][c], kv = kvv[r][c];
+          float u = su[c * T::NTL + l], ku = kuu[r][c];
+          const int reps = (q3 && c == 0) ? 2 : 1;
+          for (int rep = 0; rep < reps; ++rep) {
+            kahan(v, kv, G.acl * (a + ap));
+            kahan(u, ku, fmaf(G.dt, v, G.dsp * a));
+            ap = a;
+          }
+          gu[idx] = u; gv[idx] = v; gap[idx] = a; gku[idx] = ku; gkv[idx] = kv;
+        }
+      }
+    }
+  }
+}
+
+// Instantiated tile kernels: (dim, order, TX, TY, TZ).
+#define SEM_TILE_KERNELS(X)                                                               \
+  X(3, 2, 4, 2, 2) X(3, 3, 4, 2, 2) X(3, 4, 4, 2, 2) X(3, 5, 4, 2, 2)                     \
+  X(3, 2, 2, 2, 2) X(3, 3, 2, 2, 2) X(3, 4, 2, 2, 2) X(3, 5, 2, 2, 2)                     \
+  X(3, 3, 4, 4, 2) X(3, 4, 4, 4, 2)                                                       \
+  X(2, 2, 8, 8, 1) X(2, 3, 8, 8, 1) X(2, 4, 8, 8, 1) X(2, 5, 8, 8, 1) X(2, 6, 8, 8, 1)    \
+  X(2, 2, 16, 8, 1) X(2, 3, 16, 8, 1) X(2, 4, 16, 8, 1) X(2, 5, 16, 8, 1) X(2, 6, 16, 8, 1)
+
+}  // namespace
+
+int init_tile_tables(int device) {
+  static bool done[64] = {false};
+  if (device < 0 || device >= 64) { set_error("device index"); return -1; }
+  if (done[device]) return 0;
+  TileTables host[kTileMaxN + 1];
+  memset(host, 0, sizeof host);
+  for (int N = 1; N <= kTileMaxN; ++N) {
+    Gll g = make_gll(N);
+    for (int i = 0; i < g.n * g.n; ++i) host[N].D[i] = (float)g.D[i];
+    for (int i = 0; i < g.n; ++i) {
+      host[N].w[i] = (float)g.w[i];
+      host[N].xi[i] = (float)g.xi[i];
+    }
+  }
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(t_tab), host, sizeof host, 0, hipMemcpyHostToDevice);
+  if (e != hipSuccess) { set_error(std::string("tile tables: ") + hipGetErrorString(e)); return -2; }
+  done[device] = true;
+  return 0;
+}
+
+static bool same_shape(const int *T, int tx, int ty, int tz) {
+  return T[0] == tx && T[1] == ty && T[2] == tz;
+}
+
+TileKernelInfo tile_kernel_info(int dim, int order, int physics, const int *T) {
+  TileKernelInfo r{0, 0};
+#define TKI(D, N, X, Y, Z)                                                                  \
+  if (dim == D && order == N && same_shape(T, X, Y, D == 3 ? Z : 1)) {                      \
+    if (physics == SEM_ACOUSTIC) {                                                          \
+      using S = TS<D, N, 0, X, Y, Z>;                                                       \
+      r.lds_bytes = sizeof(float) * (size_t)S::LDS;                                         \
+      r.threads = S::BT;                                                                    \
+    } else {                                                                                \
+      using S = TS<D, N, 1, X, Y, Z>;                                                       \
+      r.lds_bytes = sizeof(float) * (size_t)S::LDS;                                         \
+      r.threads = S::BT;                                                                    \
+    }                                                                                       \
+  }
+  SEM_TILE_KERNELS(TKI)
+#undef TKI
+  return r;
+}
+
+template <int D, int N, int P, int X, int Y, int Z>
+static int go_tile(int nitems, const FusedArgs &a, hipStream_t st) {
+  using S = TS<D, N, P, X, Y, Z>;
+  const size_t shm = sizeof(float) * (size_t)S::LDS;
+  if (shm > 160 * 1024) { set_error("tile does not fit in LDS"); return -1; }
+  static bool attr = false;
+  if (!attr && shm > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_tile<D, N, P, X, Y, Z>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (e != hipSuccess) { set_error(std::string("tile LDS attribute: ") + hipGetErrorString(e)); return -2; }
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_tile<D, N, P, X, Y, Z>), dim3(nitems), dim3(S::BT), shm, st, a);
+  return 0;
+}
+
+int launch_tile(int dim, int order, int physics, const int *T, int nitems, const FusedArgs &a,
+                hipStream_t st) {
+  if (nitems <= 0) return 0;
+  int rc = 1;
+#define TLN(D, N, X, Y, Z)                                                                  \
+  if (rc == 1 && dim == D && order == N && same_shape(T, X, Y, D == 3 ? Z : 1))             \
+    rc = physics == SEM_ACOUSTIC ? go_tile<D, N, 0, X, Y, Z>(nitems, a, st)                 \
+                                 : go_tile<D, N, 1, X, Y, Z>(nitems, a, st);
+  SEM_TILE_KERNELS(TLN)
+#undef TLN
+  if (rc == 1) { set_error("tile kernel not compiled for this order/shape"); return -1; }
+  if (rc) return rc;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { set_error(std::string("tile step: ") + hipGetErrorString(e)); return -2; }
+  return 0;
+}
+
+}  // namespace sem
