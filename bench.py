@@ -1,21 +1,26 @@
 #!/usr/bin/env python3
-"""Benchmark: SEM time stepping on one MI355X.
+"""Benchmark: batched multi-shot SEM time stepping on one MI355X.
 
-Workload (default): configuration 2 -- 2-D elastic (Elastic2D) on 500x500
-order-4 quads of 10 m, 8,008,002 DOFs, vertical Ricker source, 100
-receivers, dt = 2e-4.  A "step" is one explicit Newmark time step of the
-whole wavefield (stiffness + source + mass inverse + update + receivers).
+Workload (default): configuration 4 -- 3-D elastic (Elastic3D), order-4
+hexes, 32 shots on four meshes of 32^3, 48^3, 64^3 and 96^3 elements (8 shots
+each, seeded random moment-tensor Ricker sources at 15 Hz, 3 surface
+receivers per mesh), seeded smooth vp/vs/rho with vp/vs = 1.73.  A "step"
+advances every shot by one explicit Newmark time step (stiffness + sources +
+mass inverse + update + receivers): 83.6 M nodes x 3 components x 8 shots =
+2.0 G DOF-updates per step.
 
-value      DOF-updates/s of the fused path (batched colour kernels in hipGraph
-           step blocks), inputs resident in HBM, K timed steps
-plain      the same K steps through the plain one-shot-per-launch path
-accuracy   receiver relative L2 of the FP32 HIP path against the FP64 oracle
-           on a bounded sample of the same workload (same h, material law,
-           source and receiver line on a smaller square)
-roofline   the fused kernel's algorithmic HBM bytes per launch over its
-           average duration (HIP events on the batch stream)
-cpu_baseline  the FP64 oracle (a port of the reference's CPU path) on the
-           bounded sample, 1 thread, on this host
+value        DOF-updates/s of the batched path: all shots in one set of
+             persistent tile-kernel launches per colour, hipGraph-captured
+             step blocks, fields resident in HBM, K timed steps
+plain        the same shots through the plain one-shot-per-launch path
+             (gather / stiffness / scatter / update kernels per shot)
+accuracy     receiver relative L2 of both FP32 HIP paths against the FP64
+             oracle on a bounded sample (same material law and source type on
+             a 10^3 mesh)
+roofline     the tile kernel's algorithmic HBM bytes per launch over its
+             average duration (HIP events on the batch stream)
+cpu_baseline the FP64 oracle (a port of the reference's CPU path) on the
+             bounded sample, 1 thread, on this host
 
 Prints one JSON line.
 """
@@ -35,46 +40,76 @@ METRIC = "DOF-updates/s, batched vs plain FP32 @≤1e-5 rel-L2; p50/p99 shot lat
 HBM_PEAK_GBS = 8000.0
 
 
-def build_problem(sem, W, path):
-    M = sem.Mesh(W["dim"], W["order"], W["physics"], W["cells"], W["coords"], W["par"])
-    B = sem.Batch()
-    k = B.add_shot(M, W["nsteps"], W["dt"])
-    for s in W["sources"]:
-        B.add_source(k, s["loc"], s["amp"], s["f0"], s["t0"], s["dir"])
-    for r in W["receivers"]:
-        B.add_receiver(k, r)
-    B.prepare(path)
-    return M, B, k
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def fused_bytes_per_step(M, dim, physics, order):
-    """Algorithmic HBM bytes of one fused step (all colour launches).
-
-    per element : nd (4 B gather-map word + 4c B gathered u) + per-element
-                  Jacobian factors + vertex material (affine meshes)
-    per element-node beyond a node's first/last colour: 8c B accumulator
-                  read-modify-write; first colour writes 4c, last reads 4c
-    per node    : 4 B inverse mass + c (16 B read a_, v, kv, ku + 20 B write
-                  u, v, a_, ku, kv); u comes from the gather
-    """
-    nd, c, E, G = M.nd, M.c, M.E, M.G
-    nv = 2 ** dim
-    kpar = 1 if physics == 0 else (4 if dim == 2 else 6)
+def elem_floats(dim, physics):
     ng = dim * (dim + 1) // 2 if physics == 0 else dim * dim + 1
-    per_elem = nd * (4 + 4 * c) + 4 * (ng + nv * kpar)
-    acc = 8 * c * (E * nd - G)
-    per_node = 4 + c * (16 + 20)
-    return E * per_elem + acc + G * per_node
+    kp = 1 if physics == 0 else (4 if dim == 2 else 6)
+    return ng + (2 ** dim) * kp
+
+
+def tile_bytes_per_step(M, S, dim, physics):
+    """Algorithmic HBM bytes of one step of one group on the tile kernel.
+
+    per tile      : 4 B per lattice slot word + the elements' affine factors
+                    and vertex material
+    per shot      : u gather, 4c B per mesh slot (tile halos included);
+                    accumulator 4c B per slot read (not its first colour) and
+                    per slot written (not its last colour); per node 4 B
+                    inverse mass + c (16 B read v, a_, ku, kv + 20 B write u,
+                    v, a_, ku, kv)
+    """
+    c = M.c
+    per_tile = 4 * M.tile_slots + 4 * M.ntiles * M.tile_elems * elem_floats(dim, physics)
+    per_shot = (4 * c * M.tile_valid_slots + 4 * c * (M.tile_acc_reads + M.tile_acc_writes)
+                + M.G * (4 + 36 * c))
+    return per_tile + S * per_shot
+
+
+def element_bytes_per_step(M, S, dim, physics):
+    """Algorithmic HBM bytes of one step of one group in the element-by-element
+    formulation (the operator's own data flow, independent of tiling):
+
+    per element : n^d (4 B gather index + 12c S B: u gather, a scatter
+                  read-modify-write) + the element's factors, here 4 B x
+                  (affine factors + 2^d vertex material) per element instead of
+                  per GLL point
+    per node    : S (4 B inverse mass + c (4 B a + 20 B read u, v, a_, ku, kv
+                  + 20 B write))
+    """
+    c, nd = M.c, M.nd
+    return M.E * (nd * (4 + 12 * c * S) + 4 * elem_floats(dim, physics)) + M.G * S * (4 + 44 * c)
+
+
+def make_batch(sem, meshes, W, path, nsteps):
+    B = sem.Batch()
+    shots = []
+    for M, mw in zip(meshes, W["meshes"]):
+        for s in mw["shots"]:
+            k = B.add_shot(M, nsteps, mw["dt"])
+            B.add_moment_source(k, s["loc"], s["M"], s["amp"], s["f0"], s["t0"])
+            for r in mw["receivers"]:
+                B.add_receiver(k, r)
+            shots.append(k)
+    B.prepare(path)
+    return B, shots
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--plain-steps", type=int, default=0, help="0 = same as --steps")
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--sizes", default="32,48,64,96")
+    ap.add_argument("--shots", type=int, default=8, help="shots per mesh")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--n", type=int, default=500, help="elements per side")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=400.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs one process on one MI355X; "
@@ -83,46 +118,42 @@ def main():
 
     from salvus_amd import sem, workloads
 
-    total = args.warmup + args.steps
-    W = workloads.cfg2(n=args.n, nsteps=total)
-    G = (args.n * 4 + 1) ** 2
-    dofs = 2 * G
+    sizes = tuple(int(x) for x in args.sizes.split(","))
+    W = workloads.cfg4(sizes=sizes, shots_per_mesh=args.shots)
+    t0 = time.perf_counter()
+    meshes = [sem.Mesh(3, 4, 1, mw["cells"], mw["coords"], mw["par"]) for mw in W["meshes"]]
+    log("meshes ready (%.1f s): %s" % (time.perf_counter() - t0,
+                                        ", ".join("%d^3 tiles %s" % (n, M.tile_shape)
+                                                  for n, M in zip(sizes, meshes))))
+    dofs = sum(M.c * M.G for M in meshes) * args.shots
 
-    # ---------------- fused (batched) path
-    M, B, k = build_problem(sem, W, sem.PATH_FUSED)
+    # ---------------- batched path
+    timing_steps = 10
+    total = args.warmup + args.steps + timing_steps
+    B, shots = make_batch(sem, meshes, W, sem.PATH_FUSED, total)
+    cfg = B.config()
+    log("batched path prepared: %s" % cfg)
     B.run(args.warmup)
     t0 = time.perf_counter()
     B.run(args.steps)
     fused_s = time.perf_counter() - t0
     value = dofs * args.steps / fused_s
-    cfg = B.config()
-    seis_fused = B.seismograms(k)
+    log("batched: %.1f ms/step, %.3g DOF-updates/s" % (fused_s * 1e3 / args.steps, value))
 
-    # kernel timing (eager launches bracketed by HIP events, same stream)
-    W2 = workloads.cfg2(n=args.n, nsteps=60)
-    M2, B2, _ = build_problem(sem, W2, sem.PATH_FUSED)
-    B2.run(10)
+    # kernel timing: eager launches bracketed by HIP events on the batch stream
     os.environ["SEM_TIME_KERNELS"] = "1"
-    B2.run(40)
+    B.run(timing_steps)
     os.environ["SEM_TIME_KERNELS"] = "0"
-    kms, nlaunch = B2.kernel_time()
-    step_kernel_s = kms / 1e3 / 40
-    bytes_step = fused_bytes_per_step(M2, 2, 1, 4)
-    per_launch_bytes = bytes_step / M2.ncolours
+    kms, nlaunch = B.kernel_time()
+    launches_per_step = nlaunch / timing_steps
     avg_launch_s = kms / 1e3 / max(nlaunch, 1)
+    step_bytes = sum(element_bytes_per_step(M, args.shots, 3, 1) for M in meshes)
+    per_launch_bytes = step_bytes / launches_per_step
     achieved = per_launch_bytes / avg_launch_s / 1e9
-    del B2, M2
-
-    # ---------------- plain path
-    psteps = args.plain_steps or args.steps
-    Wp = workloads.cfg2(n=args.n, nsteps=args.warmup + psteps)
-    Mp, Bp, kp = build_problem(sem, Wp, sem.PATH_PLAIN)
-    Bp.run(min(args.warmup, 20))
-    t0 = time.perf_counter()
-    Bp.run(psteps)
-    plain_s = time.perf_counter() - t0
-    plain_value = dofs * psteps / plain_s
-    del Bp, Mp
+    tile_launch_bytes = sum(tile_bytes_per_step(M, args.shots, 3, 1) for M in meshes) / launches_per_step
+    tile_achieved = tile_launch_bytes / avg_launch_s / 1e9
+    B.close()
+    del B
 
     out = {
         "metric": METRIC,
@@ -135,50 +166,92 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic (seeded Gaussian-correlated +-5% Elastic2D model)",
-        "config": {"workload": W["name"] + ", vertical Ricker 30 Hz, 100 receivers, dt 2e-4",
-                   "elements": len(W["cells"]), "order": 4, "dofs": dofs,
-                   "path": "fused", "elems_per_block": cfg["elems_per_block"],
+        "dtype": "f32",
+        "data": "synthetic (seeded smooth Elastic3D models, seeded moment tensors)",
+        "config": {"workload": W["name"] + ", order 4, moment-tensor Ricker 15 Hz, "
+                               "3 receivers per shot",
+                   "meshes": ["%d^3" % n for n in sizes], "shots": len(shots),
+                   "dof_updates_per_step": dofs, "path": "batched (tile kernel)",
+                   "tile_shapes": [list(M.tile_shape) if M.tile_shape else None for M in meshes],
+                   "shots_per_group": cfg["shots_per_group"], "groups": cfg["groups"],
                    "graph_block_steps": cfg["block_steps"]},
-        "plain": {"value": plain_value, "steps": psteps, "ms_per_step": plain_s * 1e3 / psteps},
-        "batched_vs_plain": value / plain_value,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_fused<2,4,elastic,affine>",
+                     "kernel": "k_tile<3,4,elastic,%s>" % ",".join(map(str, meshes[0].tile_shape or ())),
                      "algorithmic_bytes_per_launch": per_launch_bytes,
+                     "bytes_model": "element formulation: per element n^d (4 + 12 c S) + 4 x "
+                                    "(affine factors + vertex material); per node S (4 + 44 c)",
+                     "tile_bytes_per_launch": tile_launch_bytes,
+                     "tile_achieved_GBs": tile_achieved,
                      "avg_launch_us": avg_launch_s * 1e6,
-                     "kernel_ms_per_step": step_kernel_s * 1e3},
+                     "launches_per_step": launches_per_step,
+                     "kernel_ms_per_step": kms / timing_steps},
     }
+
+    # ---------------- plain path
+    if not args.no_plain:
+        Bp, _ = make_batch(sem, meshes, W, sem.PATH_PLAIN, 2 + args.plain_steps)
+        Bp.run(2)
+        t0 = time.perf_counter()
+        Bp.run(args.plain_steps)
+        plain_s = time.perf_counter() - t0
+        plain_value = dofs * args.plain_steps / plain_s
+        Bp.close()
+        del Bp
+        out["plain"] = {"value": plain_value, "steps": args.plain_steps,
+                        "ms_per_step": plain_s * 1e3 / args.plain_steps}
+        out["batched_vs_plain"] = value / plain_value
+        log("plain: %.1f ms/step, %.3g DOF-updates/s" % (plain_s * 1e3 / args.plain_steps, plain_value))
+    for M in meshes:
+        M.close()
 
     # ---------------- accuracy sample + CPU baseline (FP64 oracle, 1 thread)
     if not args.no_cpu:
         from oracle import pyoracle as po
         from tests.problems import rel_l2
-        Ws = workloads.cfg2(n=40, nsteps=500)
-        S = po.Sim(2, 4, 1, Ws["cells"], Ws["coords"], Ws["par"])
-        s = Ws["sources"][0]
-        S.add_source(s["loc"], s["amp"], s["f0"], s["t0"], s["dir"])
-        for r in Ws["receivers"]:
+        n, nsteps = 10, 250
+        Ws = workloads.cfg4(sizes=(n,), shots_per_mesh=1, nsteps=nsteps)
+        mw = Ws["meshes"][0]
+        s = mw["shots"][0]
+        S = po.Sim(3, 4, 1, mw["cells"], mw["coords"], mw["par"])
+        S.add_moment_source(s["loc"], s["M"], s["amp"], s["f0"], s["t0"])
+        for r in mw["receivers"]:
             S.add_receiver(r)
         t0 = time.perf_counter()
-        ref = S.run(Ws["dt"], Ws["nsteps"])
+        ref = S.run(mw["dt"], nsteps)
         cpu_s = time.perf_counter() - t0
-        Gs = (40 * 4 + 1) ** 2
-        Ms, Bs, ks = build_problem(sem, Ws, sem.PATH_FUSED)
-        Bs.run(Ws["nsteps"])
-        err = rel_l2(Bs.seismograms(ks), ref)
-        Mp2, Bp2, kp2 = build_problem(sem, Ws, sem.PATH_PLAIN)
-        Bp2.run(Ws["nsteps"])
-        err_p = rel_l2(Bp2.seismograms(kp2), ref)
-        out["accuracy"] = {"receiver_rel_l2_fused": err, "receiver_rel_l2_plain": err_p,
-                           "bar": 1e-5,
-                           "sample": "cfg2 law on 40x40 quads (10 m), 500 steps, 100 receivers"}
-        out["cpu_baseline"] = {"value": 2 * Gs * Ws["nsteps"] / cpu_s, "unit": "DOF-updates/s",
+        Ms = sem.Mesh(3, 4, 1, mw["cells"], mw["coords"], mw["par"])
+        errs = {}
+        for name, path in (("batched", sem.PATH_FUSED), ("plain", sem.PATH_PLAIN)):
+            Bs, ks = make_batch(sem, [Ms], Ws, path, nsteps)
+            Bs.run(nsteps)
+            errs[name] = rel_l2(Bs.seismograms(ks[0]), ref)
+            Bs.close()
+        Gs = (4 * n + 1) ** 3
+        out["accuracy"] = {"receiver_rel_l2_batched": errs["batched"],
+                           "receiver_rel_l2_plain": errs["plain"], "bar": 1e-5,
+                           "sample": "cfg4 law on a %d^3 mesh, 1 moment-tensor shot, %d steps, "
+                                     "3 receivers" % (n, nsteps)}
+        out["cpu_baseline"] = {"value": 3 * Gs * nsteps / cpu_s, "unit": "DOF-updates/s",
                                "cores": 1, "kind": "port",
-                               "sample": "FP64 oracle, cfg2 law on 40x40 quads, 500 steps (%.1f s)"
-                               % cpu_s}
-    print(json.dumps(out))
+                               "sample": "FP64 oracle, cfg4 law on a %d^3 mesh, %d steps (%.1f s)"
+                                         % (n, nsteps, cpu_s)}
+    # ---------------- serving sample: continuous batching vs plain FIFO
+    if not args.no_serve:
+        from salvus_amd import serve
+        specs, reqs = serve.draw_stream(args.serve_requests, args.serve_rate, 5,
+                                        models_per_category=2, size_scale=0.5, steps=(50, 300))
+        models = [serve.build_model(s["dim"], s["physics"], s["n"], s["seed"]) for s in specs]
+        log("serving models: %s" % [(s["dim"], s["physics"], s["n"]) for s in specs])
+        sb, _ = serve.run(models, reqs)
+        sp, _ = serve.run(models, reqs, path=sem.PATH_PLAIN, max_active=1)
+        out["serving"] = {
+            "batched": sb, "plain": sp,
+            "sample": "%d Poisson requests at %g/s, configuration-5 mix with sizes scaled 0.5 "
+                      "and 50-300 steps, 2 models per category" % (len(reqs), args.serve_rate)}
+        for mm in models:
+            mm["mesh"].close()
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

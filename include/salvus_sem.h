@@ -74,9 +74,11 @@ int sem_mesh_create(sem_mesh **mesh, int device, int dim, int order, int physics
                     int64_t num_cells, int64_t num_verts, const int32_t *cells,
                     const double *coords, const double *vertex_par, int quirks);
 int sem_mesh_destroy(sem_mesh *mesh);
-/* info[0..9] = {E, G (unique nodes), ncolours, components, nodes/element,
- *               affine (1/0), device bytes held, dim, tiles (0 = element
- *               kernel), tile shape Tx*10000 + Ty*100 + Tz} */
+/* info[0..15] = {E, G (unique nodes), ncolours, components, nodes/element,
+ *                affine (1/0), device bytes held, dim, tiles (0 = element
+ *                kernel), tile shape Tx*10000 + Ty*100 + Tz, tile lattice
+ *                slots, slots on the mesh, accumulator reads and writes per
+ *                shot-step, elements per tile (padded), 0} */
 int sem_mesh_info(const sem_mesh *mesh, int64_t *info);
 /* Gather map conn[e*nd + p] = global DOF, element order as given
  * (DMPlexVecGetClosure + ClsMap, Problem.cpp:302-320). */
@@ -142,6 +144,36 @@ int sem_batch_kernel_time(const sem_batch *batch, double *ms, int64_t *launches)
 /* Reference time-loop semantics (Options.cpp:84-97, Main.cpp:45): number
  * of iterations of while (t < duration) and the adjusted dt. */
 int64_t sem_reference_step_count(double duration, double dt, double *dt_out);
+
+/* ---------------------------------------------------------------- serving
+ * Shot requests on device-resident meshes with continuous batching.  No
+ * counterpart in the reference (which runs one problem per process,
+ * Main.cpp:12-75); this is the request loop the batched path serves.  A
+ * request is one shot: its wavefields live on the device from admission to
+ * completion; all active requests advance together, K steps per pump, in the
+ * same colour launches; between blocks finished requests are retired and
+ * arrived requests admitted in arrival order while the HBM budget (bytes) and
+ * the active cap (0 = none) allow.  Times are seconds on the server's clock
+ * (sem_server_clock; 0 at creation).  path SEM_PATH_PLAIN with max_active 1
+ * is the one-shot-per-launch baseline. */
+#define SEM_SOURCE_DOUBLES 19 /* kind (0 force, 1 moment), x, y, z, amp, f0, t0,
+                                 dir[3], M[3][3] row-major */
+int sem_server_create(sem_server **server, int device, int path, int64_t hbm_budget,
+                      int max_active, int block_steps);
+int sem_server_destroy(sem_server *server);
+int sem_server_clock(const sem_server *server, double *t);
+/* sources: nsrc x SEM_SOURCE_DOUBLES; receivers: nrec x 3 coordinates */
+int sem_server_submit(sem_server *server, sem_mesh *mesh, int64_t nsteps, double dt,
+                      double arrival, int nsrc, const double *sources, int nrec,
+                      const double *receivers, int *request);
+/* status[0..5] = {active, pending, finished, steps advanced, next arrival
+ *                 (-1: none), device bytes held by active requests} */
+int sem_server_pump(sem_server *server, double *status);
+/* info[0..5] = {state (0 pending, 1 active, 2 done, 3 failed), arrival,
+ *               admission, completion, steps done, device bytes} */
+int sem_server_request(const sem_server *server, int request, double *info);
+/* seismograms of a finished request: nsteps x receivers x components */
+int sem_server_result(const sem_server *server, int request, float *seis);
 
 #ifdef __cplusplus
 }

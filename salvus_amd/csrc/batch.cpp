@@ -1,0 +1,853 @@
+// Batches: shots advanced together through the plain path (one shot per
+// launch: gather / stiffness / colour scatter / update) or the batched path
+// (tile or element colour kernels over every group of the batch, in
+// hipGraph-captured step blocks).  The time loop is Main.cpp:45-66; the
+// update is Order2Newmark.cpp:68-115.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "batch.h"
+
+using namespace sem;
+
+namespace sem {
+
+int64_t field_pos(const Group &g, int64_t i) {
+  if (!g.tiled) return i;
+  const sem_mesh *m = g.mesh;
+  const int64_t c = i / m->G, node = i - c * m->G;
+  return c * m->G + m->tiles.to_internal[node];
+}
+
+void free_group(Group &g) {
+  dfree(g.fields);
+  dfree(g.d_src);
+  dfree(g.d_stf);
+  dfree(g.d_load);
+  dfree(g.d_step);
+  dfree(g.d_seis);
+  g.recs.clear();
+  g.rec_nodes.clear();
+  g.rec_w.clear();
+  g.bytes = 0;
+}
+
+void drop_graph(sem_batch *b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  if (b->graph_def) (void)hipGraphDestroy(b->graph_def);
+  b->graph = nullptr;
+  b->graph_def = nullptr;
+}
+
+// Default launch shape: elements per block so a block has ~128-256 threads.
+static int default_epb(int nd) {
+  int best = 1;
+  for (int epb = 1; epb * nd <= 512; ++epb)
+    if (block_threads(nd, epb) <= 256) best = epb;
+  return best;
+}
+
+int setup_group(sem_batch *b, int gi) {
+  Group &g = b->groups[gi];
+  sem_mesh *m = g.mesh;
+  const int path = b->path;
+  int rc = 0;
+  // tile kernel: 32-bit field offsets and at most kTileMaxSrc sources per tile
+  g.tiled = path == SEM_PATH_FUSED && m->tiles.valid &&
+            (uint64_t)m->c * (uint64_t)m->G * (uint64_t)g.S < (1ull << 32);
+  if (g.tiled) {
+    std::map<int64_t, int> per_tile;
+    for (int sl = 0; sl < g.S && g.tiled; ++sl)
+      for (auto &sp : b->shots[g.shots[sl]].srcs) {
+        int64_t e;
+        std::vector<double> ld;
+        SEM_TRY(source_load(m, sp, &e, ld));
+        if (++per_tile[m->tiles.elem_pos[e] / m->tiles.elems_per_tile] > kTileMaxSrc) {
+          g.tiled = false;
+          break;
+        }
+      }
+  }
+  const size_t len = (size_t)m->c * m->G * g.S;
+  SEM_TRY(dalloc(&g.fields, 6 * len, &g.bytes));
+  std::vector<float> buf;
+  for (int which = 0; which < 6; ++which) {
+    bool any = false;
+    if (which < 3)
+      for (int sl = 0; sl < g.S; ++sl) any |= !b->shots[g.shots[sl]].init[which].empty();
+    if (!any) {  // zero start: a fill kernel, no host buffer
+      SEM_TRY(launch_fill_f32((int64_t)len, g.fields + which * len, 0.f, b->stream));
+      continue;
+    }
+    buf.assign(len, 0.f);
+    for (int sl = 0; sl < g.S; ++sl) {
+      const Shot &s = b->shots[g.shots[sl]];
+      if (s.init[which].empty()) continue;
+      const size_t cg = (size_t)m->c * m->G;
+      for (size_t i = 0; i < cg; ++i) buf[sl * cg + field_pos(g, i)] = (float)s.init[which][i];
+    }
+    HIPCHK(hipMemcpyAsync(g.fields + which * len, buf.data(), len * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+  }
+  // step counter
+  SEM_TRY(dalloc(&g.d_step, 1, &g.bytes));
+  {
+    const int64_t zero = 0;
+    HIPCHK(hipMemcpyAsync(g.d_step, &zero, 8, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+  }
+  // sources: STF tables in FP64 at the accumulated times t_n (Main.cpp:45,
+  // Order2Newmark.cpp:112), stored FP32
+  std::vector<SrcDev> src;
+  std::vector<float> stf, load;
+  std::vector<int64_t> rec_out;  // seismogram offsets of g.recs
+  int64_t seis = 0;
+  g.rec_key = m->c | (m->nd << 8);
+  const Gll gl = make_gll(m->order);
+  for (int sl = 0; sl < g.S; ++sl) {
+    Shot &s = b->shots[g.shots[sl]];
+    s.group = gi;
+    s.slot = sl;
+    for (auto &sp : s.srcs) {
+      int64_t e;
+      std::vector<double> ld;
+      SEM_TRY(source_load(m, sp, &e, ld));
+      SrcDev d;
+      d.elem = g.tiled ? m->tiles.elem_pos[e] : m->inv_perm[e];
+      d.stf_off = (int32_t)stf.size();
+      d.load_off = (int32_t)load.size();
+      d.shot = sl;
+      double t = 0.0;
+      for (int64_t k = 0; k < g.nsteps; ++k) {
+        stf.push_back((float)ricker(t, sp.amp, sp.f0, sp.t0));
+        t += g.dt;
+      }
+      for (double v : ld) load.push_back((float)v);
+      src.push_back(d);
+    }
+    // receivers: Lagrange weights of the owning element at the receiver
+    s.seis_off = seis;
+    for (size_t r = 0; r < s.recs.size(); ++r) {
+      int64_t e;
+      double ref[3];
+      SEM_TRY(locate(m, s.recs[r].data(), &e, ref));
+      RecDev d;
+      d.nodes_off = (int32_t)g.rec_nodes.size();
+      for (int p = 0; p < m->nd; ++p) {
+        const int32_t gid = m->conn[e * m->nd + p];
+        g.rec_nodes.push_back((uint32_t)(g.tiled ? m->tiles.to_internal[gid] : gid));
+      }
+      d.w_off = (int32_t)g.rec_w.size();
+      d.shot = sl;
+      d.stride = (int32_t)(s.recs.size() * m->c);
+      d.out = nullptr;
+      rec_out.push_back(seis + (int64_t)r * m->c);
+      double l[3][kMaxN];
+      for (int a = 0; a < m->dim; ++a) lagrange(gl, ref[a], l[a]);
+      for (int p = 0; p < m->nd; ++p) {
+        const int P[3] = {p % m->n, (p / m->n) % m->n, m->dim == 3 ? p / (m->n * m->n) : 0};
+        double v = 1;
+        for (int a = 0; a < m->dim; ++a) v *= l[a][P[a]];
+        g.rec_w.push_back((float)v);
+      }
+      g.recs.push_back(d);
+    }
+    seis += (int64_t)s.recs.size() * m->c * s.nsteps;
+  }
+  g.seis_len = seis;
+  SEM_TRY(dalloc(&g.d_seis, (size_t)std::max<int64_t>(seis, 1), &g.bytes));
+  SEM_TRY(launch_fill_f32(std::max<int64_t>(seis, 1), g.d_seis, 0.f, b->stream));
+  for (size_t r = 0; r < g.recs.size(); ++r) g.recs[r].out = g.d_seis + rec_out[r];
+  g.nsrc = (int)src.size();
+  SEM_TRY(dalloc(&g.d_src, src.size(), &g.bytes));
+  SEM_TRY(dalloc(&g.d_stf, stf.size(), &g.bytes));
+  SEM_TRY(dalloc(&g.d_load, load.size(), &g.bytes));
+  if (!src.empty()) {
+    HIPCHK(hipMemcpyAsync(g.d_src, src.data(), src.size() * sizeof(SrcDev), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(g.d_stf, stf.data(), stf.size() * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(g.d_load, load.data(), load.size() * 4, hipMemcpyHostToDevice, b->stream));
+  }
+  if (path == SEM_PATH_PLAIN || (!m->affine && !g.tiled)) SEM_TRY(ensure_pointwise(m));
+  GroupDev h{};
+  h.conn = m->d_conn;
+  if (g.tiled) {
+    h.tiles = m->d_tile_info;
+    h.tile_words = m->d_tile_words;
+    h.tgeo_e = m->d_tgeo_e;
+    h.tpar_e = m->d_tpar_e;
+  }
+  h.geo_e = m->d_geo_e;
+  h.par_e = m->d_par_e;
+  h.geo_p = m->d_geo_p;
+  h.mi = g.tiled ? m->d_mi_int : m->d_mi;
+  h.u = g.fields;
+  h.v = g.fields + len;
+  h.ap = g.fields + 2 * len;
+  h.ku = g.fields + 3 * len;
+  h.kv = g.fields + 4 * len;
+  h.acc = g.fields + 5 * len;
+  h.src = g.d_src;
+  h.stf = g.d_stf;
+  h.load = g.d_load;
+  h.nsrc = g.nsrc;
+  h.S = g.S;
+  h.G = m->G;
+  h.nsteps = g.nsteps;
+  h.step = g.d_step;
+  h.dt = (float)g.dt;
+  h.acl = (float)((1.0 / 2.0) * g.dt);           // Order2Newmark.cpp:89
+  h.dsp = (float)((1.0 / 2.0) * (g.dt * g.dt));  // Order2Newmark.cpp:90
+  h.quirks = m->quirks;
+  if ((int)b->h_groups.size() <= gi) b->h_groups.resize(gi + 1);
+  b->h_groups[gi] = h;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return rc;
+}
+
+static void release_plan(sem_batch *b) {
+  for (auto &l : b->launches) dfree(l.d_items);
+  b->launches.clear();
+  for (auto &r : b->recsets) {
+    dfree(r.d_recs);
+    dfree(r.d_group);
+    dfree(r.d_nodes);
+    dfree(r.d_w);
+  }
+  b->recsets.clear();
+}
+
+static void batch_release(sem_batch *b) {
+  drop_graph(b);
+  for (auto &g : b->groups) free_group(g);
+  b->groups.clear();
+  b->h_groups.clear();
+  release_plan(b);
+  dfree(b->d_groups);
+  b->d_groups_cap = 0;
+  dfree(b->d_uloc);
+  dfree(b->d_aloc);
+  b->uloc_cap = 0;
+  for (auto e : b->ev) (void)hipEventDestroy(e);
+  b->ev.clear();
+  b->prepared = false;
+}
+
+// Work items per (kernel key, colour): tiles of tiled groups, EPB-element
+// slices of the element kernel's colour ranges otherwise.
+static int build_plan(sem_batch *b) {
+  // key = (tile?, dim, order, physics, affine, tile shape)
+  std::map<std::tuple<int, int, int, int, int, int>, std::vector<int>> keys;
+  for (int gi = 0; gi < (int)b->groups.size(); ++gi) {
+    sem_mesh *m = b->groups[gi].mesh;
+    const bool t = b->groups[gi].tiled;
+    const int shape = t ? m->tiles.T[0] * 10000 + m->tiles.T[1] * 100 + m->tiles.T[2] : 0;
+    keys[std::make_tuple((int)t, m->dim, m->order, m->physics, (int)m->affine, shape)].push_back(gi);
+  }
+  for (auto &l : b->launches) dfree(l.d_items);
+  b->launches.clear();
+  for (auto &kv : keys) {
+    const bool tile = std::get<0>(kv.first) != 0;
+    int maxcol = 0;
+    for (int gi : kv.second) {
+      sem_mesh *m = b->groups[gi].mesh;
+      maxcol = std::max(maxcol, tile ? (1 << m->dim) : m->ncol);
+    }
+    for (int c = 0; c < maxcol; ++c) {
+      std::vector<WorkItem> items;
+      for (int gi : kv.second) {
+        sem_mesh *m = b->groups[gi].mesh;
+        if (tile) {
+          const TileSet &ts = m->tiles;
+          if (c >= (int)ts.colour_start.size() - 1) continue;
+          for (int64_t k = ts.colour_start[c]; k < ts.colour_start[c + 1]; ++k) {
+            const int32_t t = ts.by_colour[k];
+            const TileInfo &ti = ts.info[t];
+            WorkItem w;
+            w.group = gi;
+            w.first = t;
+            w.count = ti.ex * ti.ey * ti.ez;
+            w.pad = 0;
+            items.push_back(w);
+          }
+        } else {
+          if (c >= m->ncol) continue;
+          for (int64_t f = m->col_start[c]; f < m->col_start[c + 1]; f += b->epb) {
+            WorkItem w;
+            w.group = gi;
+            w.first = (int32_t)f;
+            w.count = (int32_t)std::min<int64_t>(b->epb, m->col_start[c + 1] - f);
+            w.pad = 0;
+            items.push_back(w);
+          }
+        }
+      }
+      if (items.empty()) continue;
+      LaunchSet L;
+      L.tile = tile;
+      L.dim = std::get<1>(kv.first);
+      L.order = std::get<2>(kv.first);
+      L.physics = std::get<3>(kv.first);
+      L.affine = std::get<4>(kv.first) != 0;
+      L.nd = L.dim == 2 ? (L.order + 1) * (L.order + 1) : (L.order + 1) * (L.order + 1) * (L.order + 1);
+      L.c = L.physics == SEM_ACOUSTIC ? 1 : L.dim;
+      L.colour = c;
+      L.nitems = (int)items.size();
+      const int shape = std::get<5>(kv.first);
+      L.T[0] = shape / 10000;
+      L.T[1] = (shape / 100) % 100;
+      L.T[2] = shape % 100;
+      L.grid = 0;
+      if (tile) {
+        int per_cu = 1;
+        SEM_TRY(tile_kernel_prepare(L.dim, L.order, L.physics, L.T, &per_cu));
+        L.grid = L.nitems;  // one workgroup per tile
+      }
+      SEM_TRY(dalloc(&L.d_items, items.size()));
+      HIPCHK(hipMemcpyAsync(L.d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, b->stream));
+      b->launches.push_back(L);
+    }
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int finalize_groups(sem_batch *b, int epb_hint) {
+  drop_graph(b);
+  release_plan(b);
+  // group table
+  const size_t ng = b->groups.size();
+  if (ng > b->d_groups_cap) {
+    dfree(b->d_groups);
+    b->d_groups_cap = std::max<size_t>(std::max<size_t>(ng, 2 * b->d_groups_cap), 16);
+    SEM_TRY(dalloc(&b->d_groups, b->d_groups_cap));
+  }
+  if (ng)
+    HIPCHK(hipMemcpyAsync(b->d_groups, b->h_groups.data(), ng * sizeof(GroupDev), hipMemcpyHostToDevice, b->stream));
+  // receiver sets, one per (components, nodes per element)
+  std::map<int, std::vector<int>> by_key;
+  for (int gi = 0; gi < (int)ng; ++gi)
+    if (!b->groups[gi].recs.empty()) by_key[b->groups[gi].rec_key].push_back(gi);
+  for (auto &kv : by_key) {
+    std::vector<RecDev> rd;
+    std::vector<int32_t> rg;
+    std::vector<uint32_t> nodes;
+    std::vector<float> w;
+    for (int gi : kv.second) {
+      const Group &g = b->groups[gi];
+      const int32_t noff = (int32_t)nodes.size(), woff = (int32_t)w.size();
+      for (RecDev d : g.recs) {
+        d.nodes_off += noff;
+        d.w_off += woff;
+        rd.push_back(d);
+        rg.push_back(gi);
+      }
+      nodes.insert(nodes.end(), g.rec_nodes.begin(), g.rec_nodes.end());
+      w.insert(w.end(), g.rec_w.begin(), g.rec_w.end());
+    }
+    RecSet R;
+    R.C = kv.first;
+    R.nrec = (int)rd.size();
+    SEM_TRY(dalloc(&R.d_recs, rd.size()));
+    SEM_TRY(dalloc(&R.d_group, rg.size()));
+    SEM_TRY(dalloc(&R.d_nodes, nodes.size()));
+    SEM_TRY(dalloc(&R.d_w, w.size()));
+    HIPCHK(hipMemcpyAsync(R.d_recs, rd.data(), rd.size() * sizeof(RecDev), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(R.d_group, rg.data(), rg.size() * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(R.d_nodes, nodes.data(), nodes.size() * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(R.d_w, w.data(), w.size() * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->recsets.push_back(R);
+  }
+  // launch plan
+  if (b->path == SEM_PATH_PLAIN) {
+    size_t mx = 0;
+    for (auto &g : b->groups) mx = std::max(mx, (size_t)g.mesh->E * g.mesh->nd * g.mesh->c);
+    if (mx > b->uloc_cap) {
+      dfree(b->d_uloc);
+      dfree(b->d_aloc);
+      b->uloc_cap = 0;
+      SEM_TRY(dalloc(&b->d_uloc, mx));
+      SEM_TRY(dalloc(&b->d_aloc, mx));
+      b->uloc_cap = mx;
+    }
+    b->epb = 0;
+  } else {
+    int nd_max = 1;
+    for (auto &g : b->groups) nd_max = std::max(nd_max, g.mesh->nd);
+    b->epb = epb_hint > 0 ? epb_hint : default_epb(nd_max);
+    if (block_threads(nd_max, b->epb) > 512) { set_error("elements per block too large"); return -1; }
+    SEM_TRY(build_plan(b));
+    int64_t min_steps = INT64_MAX;
+    for (auto &g : b->groups) min_steps = std::min(min_steps, std::max<int64_t>(g.nsteps, 1));
+    if (b->groups.empty()) min_steps = 1;
+    b->block_steps = (int)std::min<int64_t>(std::max<int64_t>(1, 4096 / std::max<int>(1, (int)b->launches.size())), 50);
+    b->block_steps = std::max(1, std::min<int>(b->block_steps, (int)min_steps));
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+static int enqueue_receivers(sem_batch *b, int koff) {
+  for (auto &R : b->recsets) {
+    RecArgs ra;
+    ra.groups = b->d_groups;
+    ra.rec_group = R.d_group;
+    ra.recs = R.d_recs;
+    ra.nodes = R.d_nodes;
+    ra.weights = R.d_w;
+    ra.nrec = R.nrec;
+    ra.nd = R.C >> 8;
+    ra.koff = koff;
+    ra.pad = 0;
+    SEM_TRY(launch_receivers(ra, R.C & 0xff, b->stream));
+  }
+  return 0;
+}
+
+int enqueue_fused_step(sem_batch *b, int koff, int ku_only) {
+  for (auto &L : b->launches) {
+    FusedArgs a;
+    a.groups = b->d_groups;
+    a.items = L.d_items;
+    a.nitems = L.nitems;
+    a.epb = b->epb;
+    a.koff = koff;
+    a.ku_only = ku_only;
+    auto go = [&]() {
+      return L.tile ? launch_tile(L.dim, L.order, L.physics, L.T, L.grid, a, b->stream)
+                    : launch_fused(L.dim, L.order, L.physics, L.affine, L.nitems, b->epb, L.nd, L.c, a, b->stream);
+    };
+    if (b->timing && !ku_only) {
+      hipEvent_t e0, e1;
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      b->ev.push_back(e0);
+      b->ev.push_back(e1);
+      HIPCHK(hipEventRecord(e0, b->stream));
+      SEM_TRY(go());
+      HIPCHK(hipEventRecord(e1, b->stream));
+    } else {
+      SEM_TRY(go());
+    }
+  }
+  if (ku_only) return 0;
+  return enqueue_receivers(b, koff);
+}
+
+int enqueue_plain_step(sem_batch *b) {
+  for (int gi = 0; gi < (int)b->groups.size(); ++gi) {
+    Group &g = b->groups[gi];
+    GroupDev &h = b->h_groups[gi];
+    sem_mesh *m = g.mesh;
+    SEM_TRY(launch_plain_gather(m->E, m->nd, m->c, m->G, m->d_conn, h.u, b->d_uloc, b->stream));
+    PlainArgs a;
+    a.conn = m->d_conn;
+    a.geo_p = m->d_geo_p;
+    a.uloc = b->d_uloc;
+    a.aloc = b->d_aloc;
+    a.src = g.d_src;
+    a.stf = g.d_stf;
+    a.load = g.d_load;
+    a.step = g.d_step;
+    a.nsrc = g.nsrc;
+    a.epb = default_epb(m->nd);
+    a.E = m->E;
+    a.koff = 0;
+    a.pad = 0;
+    if (b->timing) {
+      hipEvent_t e0, e1;
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      b->ev.push_back(e0);
+      b->ev.push_back(e1);
+      HIPCHK(hipEventRecord(e0, b->stream));
+      SEM_TRY(launch_plain_stiffness(m->dim, m->order, m->physics, m->E, m->nd, m->c, a, b->stream));
+      HIPCHK(hipEventRecord(e1, b->stream));
+    } else {
+      SEM_TRY(launch_plain_stiffness(m->dim, m->order, m->physics, m->E, m->nd, m->c, a, b->stream));
+    }
+    SEM_TRY(launch_fill_f32((int64_t)m->c * m->G, h.acc, 0.f, b->stream));
+    for (int c = 0; c < m->ncol; ++c)
+      SEM_TRY(launch_plain_scatter(m->col_start[c], m->col_start[c + 1] - m->col_start[c], m->nd, m->c,
+                                   m->G, m->d_conn, b->d_aloc, h.acc, b->stream));
+    SEM_TRY(launch_plain_update(m->G, m->c, m->quirks, h.dt, h.acl, h.dsp, m->d_mi, h.acc, h.u, h.v,
+                                h.ap, h.ku, h.kv, b->stream));
+  }
+  SEM_TRY(enqueue_receivers(b, 0));
+  SEM_TRY(launch_advance((int)b->groups.size(), b->d_groups, 1, b->stream));
+  return 0;
+}
+
+int capture_graph(sem_batch *b) {
+  if (b->graph) return 0;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int k = 0; k < b->block_steps && !rc; ++k) rc = enqueue_fused_step(b, k, 0);
+  if (!rc) rc = launch_advance((int)b->groups.size(), b->d_groups, b->block_steps, b->stream);
+  hipGraph_t gdef = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &gdef);
+  if (rc) {
+    if (gdef) (void)hipGraphDestroy(gdef);
+    return rc;
+  }
+  HIPCHK(e);
+  b->graph_def = gdef;
+  HIPCHK(hipGraphInstantiate(&b->graph, gdef, nullptr, nullptr, 0));
+  return 0;
+}
+
+}  // namespace sem
+
+// Fused-path stiffness-only timing of the current launch shape (no state
+// changes besides the accumulator).
+static double time_shape(sem_batch *b, int reps) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1;
+  if (enqueue_fused_step(b, 0, 1)) return -1;  // warm
+  (void)hipEventRecord(e0, b->stream);
+  for (int r = 0; r < reps; ++r)
+    if (enqueue_fused_step(b, 0, 1)) return -1;
+  (void)hipEventRecord(e1, b->stream);
+  (void)hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms;
+}
+
+static Shot *get_shot(sem_batch *b, int shot) {
+  if (shot < 0 || shot >= (int)b->shots.size()) {
+    set_error("shot index out of range");
+    return nullptr;
+  }
+  return &b->shots[shot];
+}
+
+static float *field_base(sem_batch *b, const Shot &s, int which) {
+  Group &g = b->groups[s.group];
+  const size_t len = (size_t)s.mesh->c * s.mesh->G * g.S;
+  return g.fields + (size_t)which * len;
+}
+
+extern "C" {
+
+int sem_batch_create(sem_batch **out, int device) {
+  *out = nullptr;
+  SEM_TRY(set_device(device));
+  auto *b = new sem_batch();
+  b->device = device;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    set_error("stream creation failed");
+    return -2;
+  }
+  *out = b;
+  return 0;
+}
+
+int sem_batch_destroy(sem_batch *b) {
+  if (!b) return 0;
+  (void)hipSetDevice(b->device);
+  (void)hipStreamSynchronize(b->stream);
+  batch_release(b);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+int sem_batch_add_shot(sem_batch *b, sem_mesh *m, int64_t nsteps, double dt, int *shot) {
+  if (b->prepared) { set_error("batch already prepared"); return -1; }
+  if (m->device != b->device) { set_error("mesh and batch on different devices"); return -1; }
+  if (nsteps < 0 || !(dt > 0)) { set_error("nsteps >= 0 and dt > 0 required"); return -1; }
+  Shot s;
+  s.mesh = m;
+  s.nsteps = nsteps;
+  s.dt = dt;
+  b->shots.push_back(s);
+  *shot = (int)b->shots.size() - 1;
+  return 0;
+}
+
+int sem_shot_add_source(sem_batch *b, int shot, const double *loc, double amp, double f0,
+                        double t0, const double *dir) {
+  Shot *s = get_shot(b, shot);
+  if (!s) return -1;
+  if (b->prepared) { set_error("batch already prepared"); return -1; }
+  SrcSpec sp{};
+  sp.kind = 0;
+  for (int a = 0; a < s->mesh->dim; ++a) sp.loc[a] = loc[a];
+  sp.amp = amp; sp.f0 = f0; sp.t0 = t0;
+  for (int k = 0; k < s->mesh->c; ++k) sp.dir[k] = dir ? dir[k] : (k == 0 ? 1.0 : 0.0);
+  int64_t e;
+  std::vector<double> load;
+  SEM_TRY(source_load(s->mesh, sp, &e, load));  // validates placement now
+  s->srcs.push_back(sp);
+  return 0;
+}
+
+int sem_shot_add_moment_source(sem_batch *b, int shot, const double *loc, const double *M,
+                               double amp, double f0, double t0) {
+  Shot *s = get_shot(b, shot);
+  if (!s) return -1;
+  if (b->prepared) { set_error("batch already prepared"); return -1; }
+  SrcSpec sp{};
+  sp.kind = 1;
+  const int d = s->mesh->dim;
+  for (int a = 0; a < d; ++a) sp.loc[a] = loc[a];
+  for (int i = 0; i < d * d; ++i) sp.M[i] = M[i];
+  sp.amp = amp; sp.f0 = f0; sp.t0 = t0;
+  int64_t e;
+  std::vector<double> load;
+  SEM_TRY(source_load(s->mesh, sp, &e, load));
+  s->srcs.push_back(sp);
+  return 0;
+}
+
+int sem_shot_add_receiver(sem_batch *b, int shot, const double *loc) {
+  Shot *s = get_shot(b, shot);
+  if (!s) return -1;
+  if (b->prepared) { set_error("batch already prepared"); return -1; }
+  std::array<double, 3> r{0, 0, 0};
+  for (int a = 0; a < s->mesh->dim; ++a) r[a] = loc[a];
+  int64_t e;
+  double ref[3];
+  SEM_TRY(locate(s->mesh, r.data(), &e, ref));
+  s->recs.push_back(r);
+  return 0;
+}
+
+int sem_shot_set_field(sem_batch *b, int shot, int which, const double *vals) {
+  Shot *s = get_shot(b, shot);
+  if (!s) return -1;
+  if (which < 0 || which > 2) { set_error("field selector"); return -1; }
+  const size_t len = (size_t)s->mesh->c * s->mesh->G;
+  if (!b->prepared) {
+    s->init[which].assign(vals, vals + len);
+    return 0;
+  }
+  // live update of this shot's slice
+  HIPCHK(hipSetDevice(b->device));
+  Group &g = b->groups[s->group];
+  std::vector<float> buf(len);
+  float *base = field_base(b, *s, which) + (size_t)s->slot * len;
+  for (size_t i = 0; i < len; ++i) buf[field_pos(g, i)] = (float)vals[i];
+  HIPCHK(hipMemcpyAsync(base, buf.data(), len * 4, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int sem_shot_get_field(sem_batch *b, int shot, int which, double *vals) {
+  Shot *s = get_shot(b, shot);
+  if (!s) return -1;
+  if (which < 0 || which > 2) { set_error("field selector"); return -1; }
+  const size_t len = (size_t)s->mesh->c * s->mesh->G;
+  if (!b->prepared) {
+    for (size_t i = 0; i < len; ++i) vals[i] = s->init[which].empty() ? 0.0 : s->init[which][i];
+    return 0;
+  }
+  HIPCHK(hipSetDevice(b->device));
+  Group &g = b->groups[s->group];
+  std::vector<float> buf(len);
+  HIPCHK(hipMemcpyAsync(buf.data(), field_base(b, *s, which) + (size_t)s->slot * len, len * 4,
+                        hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < len; ++i) vals[i] = buf[field_pos(g, i)];
+  return 0;
+}
+
+int sem_batch_prepare(sem_batch *b, int path, int epb, int spg) {
+  if (b->prepared) { set_error("batch already prepared"); return -1; }
+  if (b->shots.empty()) { set_error("batch has no shots"); return -1; }
+  if (path != SEM_PATH_PLAIN && path != SEM_PATH_FUSED) { set_error("path"); return -1; }
+  HIPCHK(hipSetDevice(b->device));
+  b->path = path;
+  // ---- grouping: shots sharing mesh, nsteps and dt (batched path only)
+  const int max_s = path == SEM_PATH_PLAIN ? 1 : (spg > 0 ? spg : 8);
+  std::map<std::tuple<sem_mesh *, int64_t, double>, std::vector<int>> buckets;
+  for (int i = 0; i < (int)b->shots.size(); ++i)
+    buckets[std::make_tuple(b->shots[i].mesh, b->shots[i].nsteps, b->shots[i].dt)].push_back(i);
+  for (auto &kv : buckets) {
+    auto &list = kv.second;
+    for (size_t k = 0; k < list.size(); k += max_s) {
+      Group g;
+      g.mesh = std::get<0>(kv.first);
+      g.nsteps = std::get<1>(kv.first);
+      g.dt = std::get<2>(kv.first);
+      for (size_t j = k; j < std::min(list.size(), k + (size_t)max_s); ++j) g.shots.push_back(list[j]);
+      g.S = (int)g.shots.size();
+      b->groups.push_back(g);
+    }
+  }
+  auto fail = [&](int r) { batch_release(b); return r; };
+  b->h_groups.resize(b->groups.size());
+  int rc = 0;
+  for (int gi = 0; gi < (int)b->groups.size(); ++gi)
+    if ((rc = setup_group(b, gi))) return fail(rc);
+  if ((rc = finalize_groups(b, epb))) return fail(rc);
+  if (path == SEM_PATH_FUSED && epb <= 0) {
+    bool any_elem = false;
+    int nd_max = 1;
+    for (auto &g : b->groups) {
+      any_elem |= !g.tiled;
+      nd_max = std::max(nd_max, g.mesh->nd);
+    }
+    if (any_elem) {
+      // autotune elements per block of the element kernel on the
+      // stiffness-only pass: one candidate per block size (the most elements
+      // that fit in bt threads)
+      double best = 1e300;
+      int best_epb = b->epb;
+      for (int bt = 64; bt <= 512; bt += 64) {
+        const int cand = bt / nd_max;
+        if (cand < 1) continue;
+        b->epb = cand;
+        if ((rc = build_plan(b))) return fail(rc);
+        const double t = time_shape(b, 3);
+        if (t < 0) return fail(-2);
+        if (t < best * 0.98) {
+          best = t;
+          best_epb = cand;
+        }
+      }
+      b->epb = best_epb;
+      if ((rc = build_plan(b))) return fail(rc);
+    }
+  }
+  b->spg = max_s;
+  if ((rc = hip_check(hipStreamSynchronize(b->stream), "sync"))) return fail(rc);
+  b->prepared = true;
+  return 0;
+}
+
+int sem_batch_run(sem_batch *b, int64_t nsteps) {
+  if (!b->prepared) { set_error("batch not prepared"); return -1; }
+  HIPCHK(hipSetDevice(b->device));
+  for (auto e : b->ev) (void)hipEventDestroy(e);
+  b->ev.clear();
+  int64_t left = nsteps;
+  if (b->path == SEM_PATH_PLAIN) {
+    const char *tg = getenv("SEM_TIME_KERNELS");
+    b->timing = tg && tg[0] == '1';
+    for (int64_t k = 0; k < left; ++k) SEM_TRY(enqueue_plain_step(b));
+  } else {
+    const char *tg = getenv("SEM_TIME_KERNELS");
+    b->timing = tg && tg[0] == '1';
+    const char *ng = getenv("SEM_NO_GRAPH");
+    const bool graphs = !(ng && ng[0] == '1') && !b->timing;
+    while (left > 0) {
+      if (graphs && left >= b->block_steps) {
+        SEM_TRY(capture_graph(b));
+        HIPCHK(hipGraphLaunch(b->graph, b->stream));
+        left -= b->block_steps;
+      } else {
+        const int k = (int)std::min<int64_t>(left, b->block_steps);
+        for (int j = 0; j < k; ++j) SEM_TRY(enqueue_fused_step(b, j, 0));
+        SEM_TRY(launch_advance((int)b->groups.size(), b->d_groups, k, b->stream));
+        left -= k;
+      }
+    }
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->steps_done += nsteps;
+  if (!b->ev.empty()) {
+    double ms = 0;
+    for (size_t i = 0; i + 1 < b->ev.size(); i += 2) {
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, b->ev[i], b->ev[i + 1]));
+      ms += t;
+    }
+    b->kernel_ms = ms;
+    b->kernel_launches = (int64_t)b->ev.size() / 2;
+  }
+  b->timing = false;
+  return 0;
+}
+
+int sem_shot_progress(sem_batch *b, int shot, int64_t *steps, double *time) {
+  Shot *s = get_shot(b, shot);
+  if (!s) return -1;
+  int64_t st = 0;
+  if (b->prepared) {
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipMemcpy(&st, b->groups[s->group].d_step, 8, hipMemcpyDeviceToHost));
+  }
+  *steps = st;
+  double t = 0;
+  for (int64_t k = 0; k < st; ++k) t += s->dt;
+  *time = t;
+  return 0;
+}
+
+int sem_shot_get_seismograms(sem_batch *b, int shot, float *out) {
+  Shot *s = get_shot(b, shot);
+  if (!s) return -1;
+  if (!b->prepared) { set_error("batch not prepared"); return -1; }
+  const size_t n = s->recs.size() * s->mesh->c * s->nsteps;
+  if (n == 0) return 0;
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipMemcpy(out, b->groups[s->group].d_seis + s->seis_off, n * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int sem_batch_config(const sem_batch *b, int64_t *cfg) {
+  cfg[0] = b->epb;
+  cfg[1] = b->spg;
+  cfg[2] = b->block_steps;
+  cfg[3] = (int64_t)b->groups.size();
+  return 0;
+}
+
+int sem_batch_kernel_time(const sem_batch *b, double *ms, int64_t *launches) {
+  *ms = b->kernel_ms;
+  *launches = b->kernel_launches;
+  return 0;
+}
+
+int sem_mesh_apply_stiffness(sem_mesh *m, int path, const float *u, float *ku) {
+  HIPCHK(hipSetDevice(m->device));
+  sem_batch *b = nullptr;
+  SEM_TRY(sem_batch_create(&b, m->device));
+  int shot = 0;
+  int rc = sem_batch_add_shot(b, m, 1, 1.0, &shot);
+  if (!rc) rc = sem_batch_prepare(b, path, path == SEM_PATH_FUSED ? default_epb(m->nd) : 0, 1);
+  const size_t len = (size_t)m->c * m->G;
+  std::vector<float> uin(u, u + len);
+  if (!rc && b->groups[0].tiled)
+    for (size_t i = 0; i < len; ++i) uin[field_pos(b->groups[0], i)] = u[i];
+  if (!rc) rc = hip_check(hipMemcpyAsync(b->h_groups[0].u, uin.data(), len * 4, hipMemcpyHostToDevice, b->stream), "upload");
+  if (!rc) {
+    if (path == SEM_PATH_FUSED) {
+      rc = enqueue_fused_step(b, 0, 1);
+    } else {
+      GroupDev &h = b->h_groups[0];
+      rc = launch_plain_gather(m->E, m->nd, m->c, m->G, m->d_conn, h.u, b->d_uloc, b->stream);
+      PlainArgs a{};
+      a.conn = m->d_conn;
+      a.geo_p = m->d_geo_p;
+      a.uloc = b->d_uloc;
+      a.aloc = b->d_aloc;
+      a.epb = default_epb(m->nd);
+      a.E = m->E;
+      if (!rc) rc = launch_plain_stiffness(m->dim, m->order, m->physics, m->E, m->nd, m->c, a, b->stream);
+      if (!rc) rc = launch_fill_f32((int64_t)len, h.acc, 0.f, b->stream);
+      for (int c = 0; c < m->ncol && !rc; ++c)
+        rc = launch_plain_scatter(m->col_start[c], m->col_start[c + 1] - m->col_start[c], m->nd, m->c,
+                                  m->G, m->d_conn, b->d_aloc, h.acc, b->stream);
+    }
+  }
+  std::vector<float> acc(len);
+  if (!rc) rc = hip_check(hipMemcpyAsync(acc.data(), b->h_groups[0].acc, len * 4, hipMemcpyDeviceToHost, b->stream), "copy");
+  if (!rc) rc = hip_check(hipStreamSynchronize(b->stream), "sync");
+  if (!rc)
+    for (size_t i = 0; i < len; ++i) ku[i] = -acc[field_pos(b->groups[0], i)];
+  sem_batch_destroy(b);
+  return rc;
+}
+
+}  // extern "C"

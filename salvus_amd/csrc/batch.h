@@ -1,0 +1,138 @@
+// Internal host structures of batches (shots advanced together) and of the
+// serving loop that keeps a changing set of shots on the device.
+#pragma once
+
+#include <array>
+#include <string>
+#include <vector>
+
+#include "sem_internal.h"
+
+namespace sem {
+
+template <typename T>
+inline int dalloc(T **p, size_t count, size_t *bytes = nullptr) {
+  *p = nullptr;
+  if (count == 0) return 0;
+  hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    set_error(std::string("device allocation of ") + std::to_string(count * sizeof(T)) +
+              " bytes failed: " + hipGetErrorString(e));
+    *p = nullptr;
+    return -3;
+  }
+  if (bytes) *bytes += count * sizeof(T);
+  return 0;
+}
+template <typename T>
+inline void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+struct SrcSpec {
+  int kind;  // 0 force, 1 moment tensor
+  double loc[3], amp, f0, t0, dir[3], M[9];
+};
+
+struct Shot {
+  sem_mesh *mesh = nullptr;
+  int64_t nsteps = 0;
+  double dt = 0;
+  std::vector<SrcSpec> srcs;
+  std::vector<std::array<double, 3>> recs;
+  std::vector<double> init[3];
+  int group = -1, slot = 0;
+  int64_t seis_off = 0;  // within the group's seismogram buffer
+};
+
+// Shots sharing a mesh, nsteps and dt, advanced in lockstep; fields are
+// shot-major: field[(slot*c + comp)*G + node].
+struct Group {
+  sem_mesh *mesh = nullptr;
+  std::vector<int> shots;
+  int S = 1;
+  int64_t nsteps = 0;
+  double dt = 0;
+  float *fields = nullptr;  // 6 arrays of c*G*S: u v ap ku kv acc
+  SrcDev *d_src = nullptr;
+  float *d_stf = nullptr, *d_load = nullptr;
+  int nsrc = 0;
+  bool tiled = false;       // tile kernel; fields in the mesh's tile-major numbering
+  int64_t *d_step = nullptr;  // device step counter
+  float *d_seis = nullptr;    // seismograms of the group's shots
+  int64_t seis_len = 0;
+  // receivers (offsets local to the group; RecDev::out is absolute)
+  std::vector<RecDev> recs;
+  std::vector<uint32_t> rec_nodes;
+  std::vector<float> rec_w;
+  int rec_key = 0;  // components | nodes per element << 8
+  size_t bytes = 0;  // device bytes held
+};
+
+struct LaunchSet {
+  int dim, order, physics, nd, c, colour;
+  bool affine;
+  bool tile;
+  int nitems;
+  int T[3];  // tile shape (tile kernel)
+  int grid;  // tile kernel: workgroups (one per tile)
+  WorkItem *d_items;
+};
+
+struct RecSet {
+  int C = 0, nrec = 0;  // C: components | nodes per element << 8
+  RecDev *d_recs = nullptr;
+  int32_t *d_group = nullptr;
+  uint32_t *d_nodes = nullptr;
+  float *d_w = nullptr;
+};
+
+}  // namespace sem
+
+struct sem_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<sem::Shot> shots;
+  std::vector<sem::Group> groups;
+  bool prepared = false;
+  int path = SEM_PATH_FUSED, epb = 0, spg = 0, block_steps = 0;
+  sem::GroupDev *d_groups = nullptr;
+  size_t d_groups_cap = 0;
+  std::vector<sem::GroupDev> h_groups;
+  std::vector<sem::LaunchSet> launches;
+  std::vector<sem::RecSet> recsets;
+  float *d_uloc = nullptr, *d_aloc = nullptr;  // plain path scratch
+  size_t uloc_cap = 0;
+  hipGraphExec_t graph = nullptr;
+  hipGraph_t graph_def = nullptr;
+  int64_t steps_done = 0;
+  bool timing = false;
+  double kernel_ms = 0;
+  int64_t kernel_launches = 0;
+  std::vector<hipEvent_t> ev;
+};
+
+namespace sem {
+
+// api.cpp
+int set_device(int device);  // selects the device and uploads its tables once
+int ensure_pointwise(sem_mesh *m);
+int locate(const sem_mesh *m, const double *pt, int64_t *elem, double *ref);
+int source_load(const sem_mesh *m, const SrcSpec &s, int64_t *elem, std::vector<double> &load);
+double ricker(double t, double amp, double f0, double t0);  // Ricker.cpp:27-33
+// batch.cpp
+// Position of DMPlex DOF entry i (= comp*G + g) in a group's field arrays.
+int64_t field_pos(const Group &g, int64_t i);
+// Device state of b->groups[gi] from its shots (fields, sources, receivers);
+// fills b->h_groups[gi].
+int setup_group(sem_batch *b, int gi);
+void free_group(Group &g);
+// Uploads the group table, receivers and launch plan of the current groups.
+int finalize_groups(sem_batch *b, int epb_hint);
+void drop_graph(sem_batch *b);
+int enqueue_fused_step(sem_batch *b, int koff, int ku_only);
+int enqueue_plain_step(sem_batch *b);
+int capture_graph(sem_batch *b);
+
+}  // namespace sem

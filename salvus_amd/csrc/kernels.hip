@@ -471,7 +471,7 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
   for (int s = 0; s < Sg; ++s) {
     if (active) {
 #pragma unroll
-      for (int c = 0; c < C; ++c) su[c * S::ND + p] = G.u[((int64_t)c * G.G + g) * Sg + s];
+      for (int c = 0; c < C; ++c) su[c * S::ND + p] = G.u[((int64_t)s * C + c) * G.G + g];
     }
     __syncthreads();
     if (active) node_flux<DIM, N, PHYS>(X, M, su, sF);
@@ -493,7 +493,7 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
       }
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        const int64_t idx = ((int64_t)c * G.G + g) * Sg + s;
+        const int64_t idx = ((int64_t)s * C + c) * G.G + g;
         if (!(word & kFlagLast) || A.ku_only) {
           if (word & kFlagFirst) G.acc[idx] = f[c];
           else G.acc[idx] += f[c];
@@ -507,7 +507,7 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           if (q3 && c == 2) continue;
-          const int64_t idx = ((int64_t)c * G.G + g) * Sg + s;
+          const int64_t idx = ((int64_t)s * C + c) * G.G + g;
           float a = mi * f[c];
           float ap = G.ap[idx];
           float v = G.v[idx], kv = G.kv[idx];
@@ -639,15 +639,16 @@ __global__ void k_receivers(RecArgs A, int C) {
   const uint32_t *nodes = A.nodes + R.nodes_off;
   float s = 0.f;
   for (int p = 0; p < A.nd; ++p)
-    s = fmaf(A.weights[R.w_off + p], G.u[((int64_t)c * G.G + nodes[p]) * G.S + R.shot], s);
-  A.out[R.out_off + step * R.stride + c] = s;
+    s = fmaf(A.weights[R.w_off + p], G.u[((int64_t)R.shot * C + c) * G.G + nodes[p]], s);
+  R.out[step * R.stride + c] = s;
 }
 
-__global__ void k_advance(int ngroups, const GroupDev *groups, int64_t *steps, int64_t k) {
+__global__ void k_advance(int ngroups, const GroupDev *groups, int64_t k) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ngroups) return;
-  int64_t s = steps[i] + k;
-  steps[i] = s < groups[i].nsteps ? s : groups[i].nsteps;
+  int64_t *step = groups[i].step;
+  const int64_t s = *step + k;
+  *step = s < groups[i].nsteps ? s : groups[i].nsteps;
 }
 
 }  // namespace sem
@@ -825,8 +826,9 @@ int launch_receivers(const RecArgs &a, int C, hipStream_t st) {
   return last_launch("receivers");
 }
 
-int launch_advance(int ngroups, const GroupDev *groups, int64_t *steps, int64_t k, hipStream_t st) {
-  k_advance<<<nblocks(ngroups, 64), 64, 0, st>>>(ngroups, groups, steps, k);
+int launch_advance(int ngroups, const GroupDev *groups, int64_t k, hipStream_t st) {
+  if (ngroups <= 0) return 0;
+  k_advance<<<nblocks(ngroups, 64), 64, 0, st>>>(ngroups, groups, k);
   return last_launch("advance");
 }
 

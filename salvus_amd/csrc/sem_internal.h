@@ -101,7 +101,7 @@ struct RecDev {
   int32_t w_off;     // offset of nd Lagrange weights
   int32_t shot;      // shot slot within the group
   int32_t stride;    // floats between consecutive steps (R_shot * c)
-  int64_t out_off;   // offset of this receiver's step-0, component-0 sample
+  float *out;        // this receiver's step-0, component-0 sample
 };
 
 // One tile of a logically Cartesian mesh (see tiles.cpp).
@@ -113,8 +113,8 @@ struct TileInfo {
   int64_t node_off;    // first node slot
 };
 
-// One group = shots sharing a mesh, fields interleaved shot-fastest:
-// field[(comp*G + g)*S + s].
+// One group = shots sharing a mesh, fields shot-major:
+// field[(s*c + comp)*G + g] (each shot's field contiguous).
 struct GroupDev {
   const uint32_t *conn;   // E x nd (sorted order, flags)
   const float *geo_e;     // affine: E x ng
@@ -129,7 +129,7 @@ struct GroupDev {
   int32_t S;
   int64_t G;
   int64_t nsteps;
-  const int64_t *step;    // device step counter of this group
+  int64_t *step;          // device step counter of this group
   float dt, acl, dsp;
   int32_t quirks;
   int32_t pad;
@@ -199,7 +199,6 @@ struct RecArgs {
   const RecDev *recs;
   const uint32_t *nodes;
   const float *weights;
-  float *out;
   int32_t nrec;
   int32_t nd;
   int32_t koff;
@@ -239,7 +238,8 @@ int launch_plain_update(int64_t G, int C, int quirks, float dt, float acl, float
                         const float *mi, const float *acc, float *u, float *v, float *ap,
                         float *ku, float *kv, hipStream_t st);
 int launch_receivers(const RecArgs &a, int C, hipStream_t st);
-int launch_advance(int ngroups, const GroupDev *groups, int64_t *steps, int64_t k, hipStream_t st);
+// Advances every group's step counter by k (clamped at its nsteps).
+int launch_advance(int ngroups, const GroupDev *groups, int64_t k, hipStream_t st);
 
 // ---------------------------------------------------------------- tile kernel (tile_kernels.hip)
 int init_tile_tables(int device);
@@ -248,7 +248,10 @@ struct TileKernelInfo {
   int threads;
 };
 TileKernelInfo tile_kernel_info(int dim, int order, int physics, const int *T);
-int launch_tile(int dim, int order, int physics, const int *T, int nitems, const FusedArgs &a,
+// Readies the kernel of this shape (LDS limit) and returns its workgroups per CU.
+int tile_kernel_prepare(int dim, int order, int physics, const int *T, int *per_cu);
+// One workgroup per tile: grid = a.nitems.
+int launch_tile(int dim, int order, int physics, const int *T, int grid, const FusedArgs &a,
                 hipStream_t st);
 
 }  // namespace sem

@@ -1,0 +1,262 @@
+// Serving loop: shot requests on device-resident meshes with continuous
+// batching.  Each admitted request becomes one group of a long-lived batch;
+// all active groups advance together in the same colour launches (one
+// persistent tile launch per colour and kernel shape covers the tiles of
+// every active group), in blocks of K steps.  Between blocks, finished
+// requests are retired (seismograms copied out, device memory released) and
+// requests that have arrived are admitted in arrival order while the HBM
+// budget and the active-request cap allow.  Requests keep their wavefields
+// on the device from admission to completion.
+//
+// A shot's arithmetic does not depend on which other shots share its
+// launches (every group owns its fields, step counter and sources, and the
+// kernels sum in a fixed order), so a request's result is the same alone and
+// in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <deque>
+
+#include "batch.h"
+
+namespace sem {
+
+struct Request {
+  Shot shot;
+  double arrival = 0, t_admit = -1, t_done = -1;
+  int state = 0;  // 0 pending, 1 active, 2 done, 3 failed
+  int64_t bytes = 0, steps = 0;
+  std::vector<float> seis;
+};
+
+}  // namespace sem
+
+using namespace sem;
+
+struct sem_server {
+  sem_batch *b = nullptr;
+  int path = SEM_PATH_FUSED;
+  int64_t budget = 0, used = 0;
+  int max_active = 0, block_steps = 32;
+  std::vector<Request> reqs;
+  std::deque<int> pending;  // arrival order
+  std::vector<int> active;  // request of each batch group, in group order
+  std::chrono::steady_clock::time_point t0;
+  int64_t done = 0;
+};
+
+static double clock_of(const sem_server *s) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
+}
+
+// Device bytes a request holds while active.
+static int64_t request_bytes(const sem_server *s, const Shot &sh) {
+  const sem_mesh *m = sh.mesh;
+  int64_t b = 6 * 4 * (int64_t)m->c * m->G + 8;
+  b += 4 * (int64_t)sh.recs.size() * m->c * sh.nsteps;
+  b += (int64_t)sh.srcs.size() * (4 * sh.nsteps + 4 * (int64_t)m->nd * m->c + (int64_t)sizeof(SrcDev));
+  if (s->path == SEM_PATH_PLAIN) b += 2 * 4 * m->E * (int64_t)m->nd * m->c;  // gather/scatter scratch
+  return b;
+}
+
+static void retire(sem_server *s, double now) {
+  sem_batch *b = s->b;
+  for (int gi = (int)s->active.size() - 1; gi >= 0; --gi) {
+    Request &r = s->reqs[s->active[gi]];
+    if (r.steps < r.shot.nsteps) continue;
+    Group &g = b->groups[gi];
+    r.seis.resize((size_t)g.seis_len);
+    if (g.seis_len > 0 &&
+        hipMemcpy(r.seis.data(), g.d_seis, (size_t)g.seis_len * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      r.state = 3;
+    else
+      r.state = 2;
+    r.t_done = now;
+    s->used -= r.bytes;
+    ++s->done;
+    free_group(g);
+    b->groups.erase(b->groups.begin() + gi);
+    b->h_groups.erase(b->h_groups.begin() + gi);
+    s->active.erase(s->active.begin() + gi);
+  }
+}
+
+extern "C" {
+
+int sem_server_create(sem_server **out, int device, int path, int64_t hbm_budget, int max_active,
+                      int block_steps) {
+  *out = nullptr;
+  if (path != SEM_PATH_PLAIN && path != SEM_PATH_FUSED) { set_error("path"); return -1; }
+  sem_batch *b = nullptr;
+  SEM_TRY(sem_batch_create(&b, device));
+  auto *s = new sem_server();
+  s->b = b;
+  s->path = path;
+  b->path = path;
+  s->max_active = max_active;
+  s->block_steps = block_steps > 0 ? block_steps : 32;
+  if (hbm_budget > 0) {
+    s->budget = hbm_budget;
+  } else {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      sem_batch_destroy(b);
+      delete s;
+      set_error("hipMemGetInfo failed");
+      return -2;
+    }
+    const int64_t reserve = (int64_t)4 << 30;
+    s->budget = std::max<int64_t>((int64_t)fr - reserve, (int64_t)fr / 2);
+  }
+  s->t0 = std::chrono::steady_clock::now();
+  *out = s;
+  return 0;
+}
+
+int sem_server_destroy(sem_server *s) {
+  if (!s) return 0;
+  sem_batch_destroy(s->b);
+  delete s;
+  return 0;
+}
+
+int sem_server_clock(const sem_server *s, double *t) {
+  *t = clock_of(s);
+  return 0;
+}
+
+int sem_server_submit(sem_server *s, sem_mesh *m, int64_t nsteps, double dt, double arrival, int nsrc,
+                      const double *src, int nrec, const double *rec, int *request) {
+  if (m->device != s->b->device) { set_error("mesh and server on different devices"); return -1; }
+  if (nsteps < 1 || !(dt > 0)) { set_error("nsteps >= 1 and dt > 0 required"); return -1; }
+  Request r;
+  r.shot.mesh = m;
+  r.shot.nsteps = nsteps;
+  r.shot.dt = dt;
+  r.arrival = arrival;
+  for (int k = 0; k < nsrc; ++k) {
+    const double *p = src + (size_t)k * SEM_SOURCE_DOUBLES;
+    SrcSpec sp{};
+    sp.kind = (int)p[0];
+    if (sp.kind != 0 && sp.kind != 1) { set_error("source kind"); return -1; }
+    for (int a = 0; a < 3; ++a) sp.loc[a] = a < m->dim ? p[1 + a] : 0.0;
+    sp.amp = p[4];
+    sp.f0 = p[5];
+    sp.t0 = p[6];
+    for (int a = 0; a < 3; ++a) sp.dir[a] = a < m->c ? p[7 + a] : 0.0;
+    for (int i = 0; i < 9; ++i) sp.M[i] = 0.0;
+    for (int i = 0; i < m->dim; ++i)
+      for (int j = 0; j < m->dim; ++j) sp.M[i * m->dim + j] = p[10 + 3 * i + j];
+    int64_t e;
+    std::vector<double> load;
+    SEM_TRY(source_load(m, sp, &e, load));
+    r.shot.srcs.push_back(sp);
+  }
+  for (int k = 0; k < nrec; ++k) {
+    std::array<double, 3> x{0, 0, 0};
+    for (int a = 0; a < m->dim; ++a) x[a] = rec[3 * k + a];
+    int64_t e;
+    double ref[3];
+    SEM_TRY(locate(m, x.data(), &e, ref));
+    r.shot.recs.push_back(x);
+  }
+  r.bytes = request_bytes(s, r.shot);
+  if (r.bytes > s->budget) { set_error("request larger than the server's HBM budget"); return -1; }
+  const int id = (int)s->reqs.size();
+  s->reqs.push_back(std::move(r));
+  // keep the queue in arrival order (stable for equal arrivals)
+  auto it = s->pending.end();
+  while (it != s->pending.begin() && s->reqs[*(it - 1)].arrival > arrival) --it;
+  s->pending.insert(it, id);
+  *request = id;
+  return 0;
+}
+
+int sem_server_pump(sem_server *s, double *status) {
+  sem_batch *b = s->b;
+  HIPCHK(hipSetDevice(b->device));
+  const double now = clock_of(s);
+  // ---- admission in arrival order
+  bool changed = false;
+  while (!s->pending.empty()) {
+    const int id = s->pending.front();
+    Request &r = s->reqs[id];
+    if (r.arrival > now) break;
+    if (s->max_active > 0 && (int)s->active.size() >= s->max_active) break;
+    if (s->used + r.bytes > s->budget) break;
+    b->shots.push_back(r.shot);
+    Group g;
+    g.mesh = r.shot.mesh;
+    g.nsteps = r.shot.nsteps;
+    g.dt = r.shot.dt;
+    g.S = 1;
+    g.shots.push_back((int)b->shots.size() - 1);
+    b->groups.push_back(g);
+    const int gi = (int)b->groups.size() - 1;
+    const int rc = setup_group(b, gi);
+    if (rc) {
+      free_group(b->groups[gi]);
+      b->groups.pop_back();
+      if ((int)b->h_groups.size() > gi) b->h_groups.resize(gi);
+      if (rc == -3) break;  // device memory short: retry after retirements
+      return rc;
+    }
+    r.state = 1;
+    r.t_admit = now;
+    s->used += r.bytes;
+    s->active.push_back(id);
+    s->pending.pop_front();
+    changed = true;
+  }
+  if (changed) {
+    SEM_TRY(finalize_groups(b, 0));
+    b->prepared = true;
+  }
+  // ---- one block of steps for every active request
+  int64_t k = 0;
+  if (!s->active.empty()) {
+    b->block_steps = s->block_steps;
+    k = s->block_steps;
+    SEM_TRY(sem_batch_run(b, k));
+    for (int id : s->active) {
+      Request &r = s->reqs[id];
+      r.steps = std::min<int64_t>(r.shot.nsteps, r.steps + k);
+    }
+    const size_t before = s->active.size();
+    retire(s, clock_of(s));
+    if (s->active.size() != before) {
+      SEM_TRY(finalize_groups(b, 0));
+      b->prepared = true;
+    }
+  }
+  status[0] = (double)s->active.size();
+  status[1] = (double)s->pending.size();
+  status[2] = (double)s->done;
+  status[3] = (double)k;
+  status[4] = s->pending.empty() ? -1.0 : s->reqs[s->pending.front()].arrival;
+  status[5] = (double)s->used;
+  return 0;
+}
+
+int sem_server_request(const sem_server *s, int request, double *info) {
+  if (request < 0 || request >= (int)s->reqs.size()) { set_error("request index"); return -1; }
+  const Request &r = s->reqs[request];
+  info[0] = r.state;
+  info[1] = r.arrival;
+  info[2] = r.t_admit;
+  info[3] = r.t_done;
+  info[4] = (double)r.steps;
+  info[5] = (double)r.bytes;
+  return 0;
+}
+
+int sem_server_result(const sem_server *s, int request, float *seis) {
+  if (request < 0 || request >= (int)s->reqs.size()) { set_error("request index"); return -1; }
+  const Request &r = s->reqs[request];
+  if (r.state != 2) { set_error("request not finished"); return -1; }
+  std::copy(r.seis.begin(), r.seis.end(), seis);
+  return 0;
+}
+
+}  // extern "C"

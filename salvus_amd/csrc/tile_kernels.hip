@@ -36,6 +36,65 @@ namespace {
 
 constexpr int kTileMaxN = 8;
 
+// GLL points, weights and derivative matrix of order N evaluated at compile
+// time, so that every wave-uniform table entry becomes an instruction
+// literal.  Points: interior roots of P_N' by Newton from the Chebyshev-
+// Gauss-Lobatto guesses; w_j = 2 / (N (N+1) P_N(x_j)^2); D[i][m] = l'_m(x_i)
+// = P_N(x_i) / (P_N(x_m) (x_i - x_m)) off the diagonal, -+N(N+1)/4 at the
+// two end points and 0 elsewhere on the diagonal.
+constexpr double cx_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  const double x2 = x * x;
+  for (int k = 1; k < 40; ++k) {
+    term *= -x2 / ((2.0 * k - 1.0) * (2.0 * k));
+    sum += term;
+  }
+  return sum;
+}
+
+template <int N>
+struct GllC {
+  double x[N + 1], w[N + 1], D[(N + 1) * (N + 1)];
+  constexpr GllC() : x(), w(), D() {
+    const double pi = 3.14159265358979323846;
+    double PN[N + 1] = {};
+    for (int j = 0; j <= N; ++j) x[j] = -cx_cos(pi * j / N);
+    for (int it = 0; it < 60; ++it)
+      for (int j = 0; j <= N; ++j) {
+        double p0 = 1.0, p1 = x[j];
+        for (int k = 2; k <= N; ++k) {
+          const double p2 = ((2.0 * k - 1.0) * x[j] * p1 - (k - 1.0) * p0) / k;
+          p0 = p1;
+          p1 = p2;
+        }
+        PN[j] = p1;
+        if (j > 0 && j < N) x[j] -= (x[j] * p1 - p0) / ((N + 1) * p1);
+      }
+    for (int j = 0; j <= N; ++j) w[j] = 2.0 / (N * (N + 1.0) * PN[j] * PN[j]);
+    for (int i = 0; i <= N; ++i)
+      for (int m = 0; m <= N; ++m)
+        D[i * (N + 1) + m] = i != m ? PN[i] / (PN[m] * (x[i] - x[m]))
+                                    : (i == 0 ? -N * (N + 1.0) / 4.0 : (i == N ? N * (N + 1.0) / 4.0 : 0.0));
+  }
+};
+
+// Compile-time table entries (arguments must be constants after unrolling).
+template <int N>
+__device__ __forceinline__ float gD(int i, int m) {
+  constexpr GllC<N> g{};
+  return (float)g.D[i * (N + 1) + m];
+}
+template <int N>
+__device__ __forceinline__ float gW(int i) {
+  constexpr GllC<N> g{};
+  return (float)g.w[i];
+}
+template <int N>
+__device__ __forceinline__ float gX(int i) {
+  constexpr GllC<N> g{};
+  return (float)g.x[i];
+}
+
 struct TileTables {
   float D[(kTileMaxN + 1) * (kTileMaxN + 1)];  // D[i*n+m] = l'_m(xi_i)
   float w[kTileMaxN + 1];
@@ -205,7 +264,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT)) k_tile(Fus
         ok[r] = w != kSlotEmpty;
         const uint32_t g = ok[r] ? (w & kNodeMask) : 0u;
 #pragma unroll
-        for (int c = 0; c < C; ++c) uv[r][c] = ok[r] ? gu[(c * GG + g) * Sg + s] : 0.f;
+        for (int c = 0; c < C; ++c) uv[r][c] = ok[r] ? gu[(s * C + c) * GG + g] : 0.f;
       }
 #pragma unroll
       for (int r = 0; r < T::NIT; ++r) {
@@ -264,7 +323,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT)) k_tile(Fus
         }
 #pragma unroll
         for (int k = 0; k < n; ++k) {
-          const float tk = TT.xi[k], wk = TT.w[k];
+          const float tk = gX<N>(k), wk = gW<N>(k);
           float g[C][DIM];
 #pragma unroll
           for (int c = 0; c < C; ++c) {
@@ -282,7 +341,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT)) k_tile(Fus
             }
             float sz = 0.f;
 #pragma unroll
-            for (int m = 0; m < n; ++m) sz = fmaf(TT.D[k * n + m], uc[c][m], sz);
+            for (int m = 0; m < n; ++m) sz = fmaf(gD<N>(k, m), uc[c][m], sz);
             g[c][DIM - 1] = sz;
           }
           float F[C][DIM];
@@ -380,7 +439,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT)) k_tile(Fus
               for (int m = 0; m < n; ++m) ku = fmaf(Dcy[m], Fy[m * n], ku);
             }
 #pragma unroll
-            for (int m = 0; m < n; ++m) ku = fmaf(TT.D[m * n + k], Fz[c][m], ku);
+            for (int m = 0; m < n; ++m) ku = fmaf(gD<N>(m, k), Fz[c][m], ku);
             f[c][k] = dir ? 0.f : -ku;
           }
         }
@@ -456,7 +515,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT)) k_tile(Fus
         mi[r] = fin ? gmi[g] : 0.f;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-          const uint32_t idx = (c * GG + g) * Sg + s;
+          const uint32_t idx = (s * C + c) * GG + g;
           tot[r][c] = valid ? sK[(e * C + c) * ND + p] : 0.f;
           prev[r][c] = (valid && !first) ? gacc[idx] : 0.f;
           app[r][c] = fin ? gap[idx] : 0.f;
@@ -474,7 +533,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT)) k_tile(Fus
         const bool fin = (wk[r] & kFlagLast) && !A.ku_only;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-          const uint32_t idx = (c * GG + g) * Sg + s;
+          const uint32_t idx = (s * C + c) * GG + g;
           const float t = first ? tot[r][c] : prev[r][c] + tot[r][c];
           if (!fin) {
             gacc[idx] = t;
@@ -505,6 +564,16 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT)) k_tile(Fus
   X(2, 2, 8, 8, 1) X(2, 3, 8, 8, 1) X(2, 4, 8, 8, 1) X(2, 5, 8, 8, 1) X(2, 6, 8, 8, 1)    \
   X(2, 2, 16, 8, 1) X(2, 3, 16, 8, 1) X(2, 4, 16, 8, 1) X(2, 5, 16, 8, 1) X(2, 6, 16, 8, 1)
 
+template <int N>
+void fill_tables(TileTables &t) {
+  constexpr GllC<N> g{};
+  for (int i = 0; i < (N + 1) * (N + 1); ++i) t.D[i] = (float)g.D[i];
+  for (int i = 0; i <= N; ++i) {
+    t.w[i] = (float)g.w[i];
+    t.xi[i] = (float)g.x[i];
+  }
+}
+
 }  // namespace
 
 int init_tile_tables(int device) {
@@ -513,14 +582,15 @@ int init_tile_tables(int device) {
   if (done[device]) return 0;
   TileTables host[kTileMaxN + 1];
   memset(host, 0, sizeof host);
-  for (int N = 1; N <= kTileMaxN; ++N) {
-    Gll g = make_gll(N);
-    for (int i = 0; i < g.n * g.n; ++i) host[N].D[i] = (float)g.D[i];
-    for (int i = 0; i < g.n; ++i) {
-      host[N].w[i] = (float)g.w[i];
-      host[N].xi[i] = (float)g.xi[i];
-    }
-  }
+  // the same compile-time tables the kernels use as literals
+  fill_tables<1>(host[1]);
+  fill_tables<2>(host[2]);
+  fill_tables<3>(host[3]);
+  fill_tables<4>(host[4]);
+  fill_tables<5>(host[5]);
+  fill_tables<6>(host[6]);
+  fill_tables<7>(host[7]);
+  fill_tables<8>(host[8]);
   hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(t_tab), host, sizeof host, 0, hipMemcpyHostToDevice);
   if (e != hipSuccess) { set_error(std::string("tile tables: ") + hipGetErrorString(e)); return -2; }
   done[device] = true;
@@ -550,30 +620,53 @@ TileKernelInfo tile_kernel_info(int dim, int order, int physics, const int *T) {
   return r;
 }
 
+// Sets the dynamic-LDS limit of a kernel above 64 KiB and returns how many
+// of its workgroups one CU holds.
 template <int D, int N, int P, int X, int Y, int Z>
-static int go_tile(int nitems, const FusedArgs &a, hipStream_t st) {
+static int prep_tile(int *per_cu) {
   using S = TS<D, N, P, X, Y, Z>;
   const size_t shm = sizeof(float) * (size_t)S::LDS;
   if (shm > 160 * 1024) { set_error("tile does not fit in LDS"); return -1; }
-  static bool attr = false;
-  if (!attr && shm > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void *)k_tile<D, N, P, X, Y, Z>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+  const void *fn = (const void *)k_tile<D, N, P, X, Y, Z>;
+  if (shm > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) { set_error(std::string("tile LDS attribute: ") + hipGetErrorString(e)); return -2; }
-    attr = true;
   }
-  hipLaunchKernelGGL((k_tile<D, N, P, X, Y, Z>), dim3(nitems), dim3(S::BT), shm, st, a);
+  int nb = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, S::BT, shm);
+  if (e != hipSuccess) { set_error(std::string("tile occupancy: ") + hipGetErrorString(e)); return -2; }
+  *per_cu = nb > 0 ? nb : 1;
   return 0;
 }
 
-int launch_tile(int dim, int order, int physics, const int *T, int nitems, const FusedArgs &a,
+int tile_kernel_prepare(int dim, int order, int physics, const int *T, int *per_cu) {
+  int rc = 1;
+#define TPR(D, N, X, Y, Z)                                                                  \
+  if (rc == 1 && dim == D && order == N && same_shape(T, X, Y, D == 3 ? Z : 1))             \
+    rc = physics == SEM_ACOUSTIC ? prep_tile<D, N, 0, X, Y, Z>(per_cu)                      \
+                                 : prep_tile<D, N, 1, X, Y, Z>(per_cu);
+  SEM_TILE_KERNELS(TPR)
+#undef TPR
+  if (rc == 1) { set_error("tile kernel not compiled for this order/shape"); return -1; }
+  return rc;
+}
+
+template <int D, int N, int P, int X, int Y, int Z>
+static int go_tile(int grid, const FusedArgs &a, hipStream_t st) {
+  using S = TS<D, N, P, X, Y, Z>;
+  const size_t shm = sizeof(float) * (size_t)S::LDS;
+  hipLaunchKernelGGL((k_tile<D, N, P, X, Y, Z>), dim3(grid), dim3(S::BT), shm, st, a);
+  return 0;
+}
+
+int launch_tile(int dim, int order, int physics, const int *T, int grid, const FusedArgs &a,
                 hipStream_t st) {
-  if (nitems <= 0) return 0;
+  if (a.nitems <= 0 || grid <= 0) return 0;
   int rc = 1;
 #define TLN(D, N, X, Y, Z)                                                                  \
   if (rc == 1 && dim == D && order == N && same_shape(T, X, Y, D == 3 ? Z : 1))             \
-    rc = physics == SEM_ACOUSTIC ? go_tile<D, N, 0, X, Y, Z>(nitems, a, st)                 \
-                                 : go_tile<D, N, 1, X, Y, Z>(nitems, a, st);
+    rc = physics == SEM_ACOUSTIC ? go_tile<D, N, 0, X, Y, Z>(grid, a, st)                   \
+                                 : go_tile<D, N, 1, X, Y, Z>(grid, a, st);
   SEM_TILE_KERNELS(TLN)
 #undef TLN
   if (rc == 1) { set_error("tile kernel not compiled for this order/shape"); return -1; }

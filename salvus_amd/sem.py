@@ -36,8 +36,10 @@ EXPORTS = (
     "sem_shot_add_moment_source", "sem_shot_add_receiver", "sem_shot_set_field",
     "sem_shot_get_field", "sem_batch_prepare", "sem_batch_run", "sem_shot_progress",
     "sem_shot_get_seismograms", "sem_batch_config", "sem_batch_kernel_time",
-    "sem_reference_step_count",
+    "sem_reference_step_count", "sem_server_create", "sem_server_destroy", "sem_server_clock",
+    "sem_server_submit", "sem_server_pump", "sem_server_request", "sem_server_result",
 )
+SOURCE_DOUBLES = 19
 
 _lib = None
 _D = C.POINTER(C.c_double)
@@ -90,6 +92,15 @@ def lib():
         L.sem_batch_kernel_time.argtypes = [_VP, _D, _I64]
         L.sem_reference_step_count.restype = C.c_int64
         L.sem_reference_step_count.argtypes = [C.c_double, C.c_double, _D]
+        L.sem_server_create.argtypes = [C.POINTER(_VP), C.c_int, C.c_int, C.c_int64, C.c_int,
+                                        C.c_int]
+        L.sem_server_destroy.argtypes = [_VP]
+        L.sem_server_clock.argtypes = [_VP, _D]
+        L.sem_server_submit.argtypes = [_VP, _VP, C.c_int64, C.c_double, C.c_double, C.c_int, _D,
+                                        C.c_int, _D, C.POINTER(C.c_int)]
+        L.sem_server_pump.argtypes = [_VP, _D]
+        L.sem_server_request.argtypes = [_VP, C.c_int, _D]
+        L.sem_server_result.argtypes = [_VP, C.c_int, _F]
         _lib = L
     return _lib
 
@@ -159,7 +170,7 @@ class Mesh:
         self._h = _VP()
         _check(lib().sem_mesh_create(C.byref(self._h), device, dim, order, physics, self.E, self.V,
                                      _i(cells), _d(coords), _d(par), quirks))
-        info = np.zeros(10, dtype=np.int64)
+        info = np.zeros(16, dtype=np.int64)
         lib().sem_mesh_info(self._h, _l(info))
         self.G, self.ncolours, self.c, self.nd = (int(x) for x in info[1:5])
         self.affine = bool(info[5])
@@ -167,6 +178,9 @@ class Mesh:
         self.ntiles = int(info[8])
         t = int(info[9])
         self.tile_shape = (t // 10000, (t // 100) % 100, t % 100) if t else None
+        self.tile_slots, self.tile_valid_slots = int(info[10]), int(info[11])
+        self.tile_acc_reads, self.tile_acc_writes = int(info[12]), int(info[13])
+        self.tile_elems = int(info[14])
 
     def close(self):
         if getattr(self, "_h", None):
@@ -339,3 +353,96 @@ class Order2Newmark:
 
     def time(self):
         return self.batch.progress(self.shot)[1]
+
+
+def source_record(src):
+    """One SOURCE_DOUBLES record: kind (0 force, 1 moment tensor), location,
+    amplitude, Ricker f0 and t0, force direction, moment tensor (row-major)."""
+    rec = np.zeros(SOURCE_DOUBLES)
+    moment = src.get("kind", "force") == "moment"
+    rec[0] = 1.0 if moment else 0.0
+    loc = np.asarray(src["loc"], dtype=np.float64)
+    rec[1:1 + len(loc)] = loc
+    rec[4], rec[5], rec[6] = src.get("amp", 1.0), src["f0"], src["t0"]
+    if moment:
+        M = np.asarray(src["M"], dtype=np.float64)
+        d = M.shape[0]
+        for i in range(d):
+            for j in range(d):
+                rec[10 + 3 * i + j] = M[i, j]
+    else:
+        d = np.asarray(src.get("dir", (1.0,)), dtype=np.float64)
+        rec[7:7 + len(d)] = d
+    return rec
+
+
+class Server:
+    """Continuous-batching request loop over device-resident meshes: each
+    request is one shot; active shots advance together block_steps steps per
+    pump(); finished ones are retired and arrived ones admitted in arrival
+    order while the HBM budget (bytes, 0 = free memory less a reserve) and
+    max_active (0 = no cap) allow."""
+
+    STATES = {0: "pending", 1: "active", 2: "done", 3: "failed"}
+
+    def __init__(self, path=PATH_FUSED, hbm_budget=0, max_active=0, block_steps=32, device=0):
+        self._h = _VP()
+        _check(lib().sem_server_create(C.byref(self._h), device, path, int(hbm_budget),
+                                       max_active, block_steps))
+        self.requests = []  # (mesh, nsteps, nrec)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sem_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def clock(self):
+        t = np.zeros(1)
+        _check(lib().sem_server_clock(self._h, _d(t)))
+        return float(t[0])
+
+    def submit(self, mesh, nsteps, dt, sources, receivers, arrival=0.0):
+        src = np.ascontiguousarray(np.array([source_record(s) for s in sources]).reshape(-1))
+        if src.size == 0:
+            src = np.zeros(SOURCE_DOUBLES)
+        rec = np.zeros((max(len(receivers), 1), 3))
+        for k, r in enumerate(receivers):
+            rec[k, :len(r)] = r
+        rec = np.ascontiguousarray(rec.reshape(-1))
+        k = C.c_int(0)
+        _check(lib().sem_server_submit(self._h, mesh.handle, int(nsteps), float(dt), float(arrival),
+                                       len(sources), _d(src), len(receivers), _d(rec),
+                                       C.byref(k)))
+        self.requests.append((mesh, int(nsteps), len(receivers)))
+        return k.value
+
+    def pump(self):
+        st = np.zeros(6)
+        _check(lib().sem_server_pump(self._h, _d(st)))
+        return dict(active=int(st[0]), pending=int(st[1]), done=int(st[2]), steps=int(st[3]),
+                    next_arrival=float(st[4]), bytes=int(st[5]))
+
+    def request(self, rid):
+        info = np.zeros(6)
+        _check(lib().sem_server_request(self._h, rid, _d(info)))
+        return dict(state=self.STATES[int(info[0])], arrival=info[1], admit=info[2], done=info[3],
+                    steps=int(info[4]), bytes=int(info[5]))
+
+    def result(self, rid):
+        mesh, nsteps, nrec = self.requests[rid]
+        out = np.zeros(max(nsteps * nrec * mesh.c, 1), dtype=np.float32)
+        _check(lib().sem_server_result(self._h, rid, _f(out)))
+        return out[: nsteps * nrec * mesh.c].reshape(nsteps, nrec, mesh.c)
+
+    def drain(self, idle_sleep=0.0005):
+        """Pumps until every submitted request is finished."""
+        import time
+        while True:
+            st = self.pump()
+            if st["active"] == 0 and st["pending"] == 0:
+                return
+            if st["active"] == 0 and st["next_arrival"] > 0:
+                time.sleep(max(0.0, min(idle_sleep, st["next_arrival"] - self.clock())))

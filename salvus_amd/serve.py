@@ -1,0 +1,158 @@
+"""Serving workload (configuration 5) and its driver.
+
+A Poisson stream of shot requests at a fixed arrival rate over a mix of
+models: 40 % 2-D acoustic quads (100^2-800^2 elements), 30 % 2-D elastic
+quads (200^2-1000^2) and 30 % 3-D acoustic/elastic hexes (32^3-96^3), order 4,
+500-3000 steps each, sizes drawn log-uniformly from a seed.  Models are
+device-resident meshes (a model pool per category, each size drawn
+log-uniformly; requests pick a model of their category); a request is one
+shot (Ricker force or moment-tensor source at a seeded position, 3 receivers).
+
+run() serves the stream through sem.Server and reports per-request latency
+(completion - arrival: p50, p99) and aggregate DOF-updates/s (all requests'
+node-component updates over the makespan), for the batched server
+(continuous batching, all active requests in the same launches) or the plain
+one-shot-per-launch server (one request at a time, FIFO).
+
+python -m salvus_amd.serve [--requests N] [--rate R] [--scale S] ...
+"""
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import meshgen, sem, workloads
+
+CATEGORIES = (  # share, dim, physics (None: acoustic or elastic), size range
+    (0.4, 2, 0, (100, 800)),
+    (0.3, 2, 1, (200, 1000)),
+    (0.3, 3, None, (32, 96)),
+)
+H2, H3 = 10.0, 15.0
+
+
+def _loguniform(rng, lo, hi):
+    return int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+
+
+def build_model(dim, physics, n, seed):
+    """Mesh, dt and receivers of one model (seeded smooth material)."""
+    rng = np.random.default_rng(seed)
+    if dim == 2:
+        L = n * H2
+        cells, coords = meshgen.box_quad(n, n, L, L)
+        d = 0.05 * workloads.gaussian_field((n + 1, n + 1), 50.0 / H2, rng).ravel()
+        vp = 3000.0 * (1 + d)
+        if physics == 0:
+            par = vp[cells][..., None]
+        else:
+            vs, rho = vp / 1.732, 2500.0 * (1 + d)
+            c11 = rho * vp ** 2
+            par = np.stack([rho, c11, rho * (vp ** 2 - 2 * vs ** 2), c11, rho * vs ** 2],
+                           axis=1)[cells]
+        dt = 0.4 * H2 * workloads.GLL4_MIN_SPACING / vp.max()
+        recs = [(0.25 * L, 0.75 * L), (0.5 * L, 0.75 * L), (0.75 * L, 0.75 * L)]
+    else:
+        if physics == 0:
+            L = n * H3
+            cells, coords = meshgen.box_hex(n, n, n, L, L, L)
+            d = 0.03 * workloads.gaussian_field((n + 1,) * 3, 100.0 / H3, rng).ravel()
+            vp = (2000.0 + 3000.0 * coords[:, 2] / L) * (1 + d)
+            par = vp[cells][..., None]
+            dt = 0.4 * H3 * workloads.GLL4_MIN_SPACING / vp.max()
+        else:
+            cells, coords, par, dt, L = workloads.elastic3d_mesh(n, H3, seed)
+        recs = [(0.5 * L, 0.5 * L, L), (0.25 * L, 0.75 * L, L), (0.75 * L, 0.25 * L, L)]
+    M = sem.Mesh(dim, 4, physics, cells, coords, par)
+    return dict(mesh=M, dim=dim, physics=physics, n=n, L=L, dt=dt, recs=recs)
+
+
+def draw_stream(nreq, rate, seed, models_per_category=4, size_scale=1.0, steps=(500, 3000)):
+    """Model specs and the request stream (arrival times in seconds)."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for ci, (share, dim, phys, (lo, hi)) in enumerate(CATEGORIES):
+        for k in range(models_per_category):
+            p = phys if phys is not None else int(k % 2)
+            n = max(2, int(round(_loguniform(rng, lo, hi) * size_scale)))
+            specs.append(dict(category=ci, dim=dim, physics=p, n=n, seed=seed * 100 + len(specs)))
+    shares = np.array([c[0] for c in CATEGORIES])
+    t = 0.0
+    reqs = []
+    for i in range(nreq):
+        t += rng.exponential(1.0 / rate)
+        ci = int(rng.choice(len(CATEGORIES), p=shares))
+        models = [j for j, s in enumerate(specs) if s["category"] == ci]
+        reqs.append(dict(model=int(rng.choice(models)), arrival=t,
+                         nsteps=int(rng.integers(steps[0], steps[1] + 1)),
+                         pos=rng.uniform(0.3, 0.7, 3), mt=rng.normal(0.0, 1.0, (3, 3)),
+                         seed=int(rng.integers(1 << 30))))
+    return specs, reqs
+
+
+def _sources(model, req):
+    L, dim = model["L"], model["dim"]
+    loc = tuple(req["pos"][:dim] * L)
+    f0 = 3000.0 / (8 * (H2 if dim == 2 else H3)) / 2
+    if model["physics"] == 1 and dim == 3:
+        M = 0.5 * (req["mt"] + req["mt"].T) * 1e15
+        return [dict(kind="moment", loc=loc, M=M, amp=1.0, f0=f0, t0=1.2 / f0)]
+    c = 1 if model["physics"] == 0 else dim
+    direction = (0.0, 1.0)[:c] if c == 2 else (1.0,)
+    return [dict(kind="force", loc=loc, amp=1e6, f0=f0, t0=1.2 / f0, dir=direction)]
+
+
+def run(models, reqs, path=sem.PATH_FUSED, max_active=0, block_steps=32, hbm_budget=0):
+    """Serves the stream in real time; returns per-request latencies and the
+    aggregate throughput."""
+    S = sem.Server(path=path, hbm_budget=hbm_budget, max_active=max_active,
+                   block_steps=block_steps)
+    t_start = S.clock()
+    ids = []
+    for r in reqs:
+        m = models[r["model"]]
+        ids.append(S.submit(m["mesh"], r["nsteps"], m["dt"], _sources(m, r), m["recs"],
+                            arrival=t_start + r["arrival"]))
+    S.drain()
+    lat, dofs, t_last = [], 0, t_start
+    for r, i in zip(reqs, ids):
+        info = S.request(i)
+        if info["state"] != "done":
+            raise sem.SemError("request %d ended in state %s" % (i, info["state"]))
+        lat.append(info["done"] - info["arrival"])
+        m = models[r["model"]]["mesh"]
+        dofs += m.c * m.G * r["nsteps"]
+        t_last = max(t_last, info["done"])
+    results = [S.result(i) for i in ids]
+    S.close()
+    lat = np.array(lat)
+    span = t_last - (t_start + reqs[0]["arrival"])
+    return dict(p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                mean_s=float(lat.mean()), dof_updates_per_s=dofs / span, makespan_s=span,
+                requests=len(reqs), dof_updates=dofs), results
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--rate", type=float, default=20.0, help="arrivals per second")
+    ap.add_argument("--scale", type=float, default=1.0, help="model size scale")
+    ap.add_argument("--steps", default="500,3000")
+    ap.add_argument("--models", type=int, default=4, help="models per category")
+    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--plain", action="store_true", help="also serve through the plain path")
+    args = ap.parse_args()
+    lo, hi = (int(x) for x in args.steps.split(","))
+    specs, reqs = draw_stream(args.requests, args.rate, args.seed, args.models, args.scale,
+                              (lo, hi))
+    models = [build_model(s["dim"], s["physics"], s["n"], s["seed"]) for s in specs]
+    out = {"batched": run(models, reqs)[0]}
+    if args.plain:
+        out["plain"] = run(models, reqs, path=sem.PATH_PLAIN, max_active=1)[0]
+    out["models"] = [dict(dim=s["dim"], physics=s["physics"], n=s["n"]) for s in specs]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

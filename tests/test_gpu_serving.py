@@ -1,0 +1,128 @@
+"""Serving loop on the GPU: continuous batching of shot requests.
+
+Bars: every request's seismograms are the same (bit for bit) alone and in a
+mix of concurrently active requests on different meshes and physics;
+requests arriving while others run are admitted between blocks and overlap
+them on the device; admission respects the HBM budget; a served shot agrees
+with the FP64 oracle within the run tolerance; the plain one-shot-per-launch
+server agrees with the batched one within the same tolerance."""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from salvus_amd import sem
+from tests import problems as pb
+
+pytestmark = pytest.mark.gpu
+
+TOL_RUN = 1e-5
+
+# dim, physics, elements per side, steps
+CASES = [(2, 0, 12, 90), (2, 1, 10, 70), (3, 0, 4, 60), (3, 1, 4, 50)]
+
+
+def _problems():
+    return [pb.make(d, ph, nel, order=4, seed=30 + i, nsteps=ns)
+            for i, (d, ph, nel, ns) in enumerate(CASES)]
+
+
+def _mesh(P):
+    return sem.Mesh(P["dim"], P["order"], P["physics"], P["cells"], P["coords"], P["par"])
+
+
+def _sources(P, scale=1.0):
+    s = P["src"]
+    return [dict(kind="force", loc=np.asarray(s["loc"]) * scale, amp=s["amp"], f0=s["f0"],
+                 t0=s["t0"], dir=s["dir"])]
+
+
+def _alone(M, P, scale=1.0, **kw):
+    S = sem.Server(block_steps=16, **kw)
+    r = S.submit(M, P["nsteps"], P["dt"], _sources(P, scale), P["recs"])
+    S.drain()
+    assert S.request(r)["state"] == "done"
+    out = S.result(r)
+    S.close()
+    return out
+
+
+def test_results_same_alone_and_in_mix():
+    Ps = _problems()
+    Ms = [_mesh(P) for P in Ps]
+    shots = [(M, P, 1.0) for M, P in zip(Ms, Ps)] + [(Ms[0], Ps[0], 0.9), (Ms[3], Ps[3], 1.1)]
+    alone = [_alone(M, P, f) for M, P, f in shots]
+    S = sem.Server(block_steps=16)
+    ids = [S.submit(M, P["nsteps"], P["dt"], _sources(P, f), P["recs"]) for M, P, f in shots]
+    st = S.pump()
+    assert st["active"] == len(shots)  # all admitted in the first block
+    S.drain()
+    for rid, a in zip(ids, alone):
+        info = S.request(rid)
+        assert info["state"] == "done" and info["steps"] == S.requests[rid][1]
+        np.testing.assert_array_equal(S.result(rid), a)
+    assert np.abs(alone[0]).max() > 0
+
+
+def test_late_arrivals_join_running_requests():
+    """A request arriving while another runs is admitted at the next block
+    boundary and finishes while the first one is still active or after it,
+    with the same result as alone."""
+    Ps = _problems()
+    M0, M2 = _mesh(Ps[0]), _mesh(Ps[2])
+    ref = _alone(M2, Ps[2])
+    S = sem.Server(block_steps=8)
+    a = S.submit(M0, 400, Ps[0]["dt"], _sources(Ps[0]), Ps[0]["recs"])
+    S.pump()
+    b = S.submit(M2, Ps[2]["nsteps"], Ps[2]["dt"], _sources(Ps[2]), Ps[2]["recs"],
+                 arrival=S.clock())
+    S.drain()
+    ia, ib = S.request(a), S.request(b)
+    assert ib["admit"] > ia["admit"] and ib["admit"] < ia["done"]
+    np.testing.assert_array_equal(S.result(b), ref)
+
+
+def test_admission_by_hbm_budget():
+    """With a budget that holds one request, requests run one after another;
+    results are unchanged."""
+    Ps = _problems()
+    M = _mesh(Ps[2])
+    probe = sem.Server(block_steps=16)
+    r = probe.submit(M, Ps[2]["nsteps"], Ps[2]["dt"], _sources(Ps[2]), Ps[2]["recs"])
+    need = probe.request(r)["bytes"]
+    probe.close()
+    ref = _alone(M, Ps[2])
+    S = sem.Server(block_steps=16, hbm_budget=int(need * 1.5))
+    ids = [S.submit(M, Ps[2]["nsteps"], Ps[2]["dt"], _sources(Ps[2]), Ps[2]["recs"])
+           for _ in range(3)]
+    peak = 0
+    while True:
+        st = S.pump()
+        peak = max(peak, st["active"])
+        assert st["bytes"] <= need * 1.5
+        if st["active"] == 0 and st["pending"] == 0:
+            break
+    assert peak == 1
+    infos = [S.request(i) for i in ids]
+    assert infos[0]["done"] <= infos[1]["admit"] and infos[1]["done"] <= infos[2]["admit"]
+    for i in ids:
+        np.testing.assert_array_equal(S.result(i), ref)
+    with pytest.raises(sem.SemError):
+        S.submit(M, 10 ** 9, Ps[2]["dt"], _sources(Ps[2]), Ps[2]["recs"])  # over the budget
+
+
+@pytest.mark.parametrize("case", [1, 3])
+def test_served_shot_matches_oracle(case):
+    P = _problems()[case]
+    M = _mesh(P)
+    O = po.Sim(P["dim"], P["order"], P["physics"], P["cells"], P["coords"], P["par"])
+    s = P["src"]
+    O.add_source(s["loc"], s["amp"], s["f0"], s["t0"], s["dir"])
+    for r in P["recs"]:
+        O.add_receiver(r)
+    ref = O.run(P["dt"], P["nsteps"])
+    got = _alone(M, P)
+    assert pb.rel_l2(got, ref) < TOL_RUN
+    plain = _alone(M, P, path=sem.PATH_PLAIN, max_active=1)
+    assert pb.rel_l2(plain, ref) < TOL_RUN
