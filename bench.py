@@ -1,32 +1,42 @@
 #!/usr/bin/env python3
-"""Benchmark: batched multi-shot SEM time stepping on one MI355X.
+"""Benchmark: batched SEM time stepping on one MI355X.
 
-Workload (default): configuration 4 -- 3-D elastic (Elastic3D), order-4
-hexes, 32 shots on four meshes of 32^3, 48^3, 64^3 and 96^3 elements (8 shots
-each, seeded random moment-tensor Ricker sources at 15 Hz, 3 surface
-receivers per mesh), seeded smooth vp/vs/rho with vp/vs = 1.73.  A "step"
-advances every shot by one explicit Newmark time step (stiffness + sources +
-mass inverse + update + receivers): 83.6 M nodes x 3 components x 8 shots =
-2.0 G DOF-updates per step.
+Workloads (--workload):
+  cfg4 (default)  3-D elastic (Elastic3D), order-4 hexes, 32 shots on four
+                  meshes of 32^3, 48^3, 64^3 and 96^3 elements (8 shots each,
+                  seeded moment-tensor Ricker sources at 15 Hz, 3 surface
+                  receivers per shot), seeded smooth vp/vs/rho, vp/vs = 1.73:
+                  2.0 G DOF-updates per step
+  cfg3            3-D acoustic, 128^3 order-4 hexes (135 M nodes), layered vp,
+                  Ricker 20 Hz, 1024 surface receivers, one shot
+  cfg2            2-D elastic, 500^2 order-4 quads (8 M DOFs), vertical force,
+                  100 receivers, one shot
+A "step" advances every shot by one explicit Newmark time step (stiffness +
+sources + mass inverse + update + receivers).
 
-value        DOF-updates/s of the batched path: all shots in one set of
-             persistent tile-kernel launches per colour, hipGraph-captured
-             step blocks, fields resident in HBM, K timed steps
+value        DOF-updates/s of the batched path: every shot in one set of tile
+             kernel launches per colour, hipGraph-captured step blocks, fields
+             resident in HBM, K timed steps
 plain        the same shots through the plain one-shot-per-launch path
              (gather / stiffness / scatter / update kernels per shot)
 accuracy     receiver relative L2 of both FP32 HIP paths against the FP64
-             oracle on a bounded sample (same material law and source type on
-             a 10^3 mesh)
+             oracle on a bounded sample of the workload (same material law,
+             source type and receiver layout on a smaller mesh)
 roofline     the tile kernel's algorithmic HBM bytes per launch over its
-             average duration (HIP events on the batch stream)
+             average duration (HIP events on the batch stream); traffic from
+             the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the
+             same workload when present
 cpu_baseline the FP64 oracle (a port of the reference's CPU path) on the
              bounded sample, 1 thread, on this host
+serving      (cfg4) a configuration-5 request stream through the
+             continuous-batching server and the plain FIFO server
 
 Prints one JSON line.
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -83,31 +93,82 @@ def element_bytes_per_step(M, S, dim, physics):
     return M.E * (nd * (4 + 12 * c * S) + 4 * elem_floats(dim, physics)) + M.G * S * (4 + 44 * c)
 
 
-def make_batch(sem, meshes, W, path, nsteps):
+def setup(sem, workloads, name, nsteps, sizes, shots_per_mesh):
+    """(dim, physics, meshes, shot specs, description) of a workload; a shot
+    spec is (mesh index, dt, sources, receivers)."""
+    if name == "cfg4":
+        W = workloads.cfg4(sizes=sizes, shots_per_mesh=shots_per_mesh, nsteps=nsteps)
+        meshes = [sem.Mesh(3, 4, 1, mw["cells"], mw["coords"], mw["par"]) for mw in W["meshes"]]
+        shots = [(i, mw["dt"], [s], mw["receivers"])
+                 for i, mw in enumerate(W["meshes"]) for s in mw["shots"]]
+        return 3, 1, meshes, shots, W["name"] + ", order 4, moment-tensor Ricker 15 Hz, " \
+                                                "3 receivers per shot"
+    W = workloads.cfg3(nsteps=nsteps) if name == "cfg3" else workloads.cfg2(nsteps=nsteps)
+    meshes = [sem.Mesh(W["dim"], W["order"], W["physics"], W["cells"], W["coords"], W["par"])]
+    return W["dim"], W["physics"], meshes, [(0, W["dt"], W["sources"], W["receivers"])], \
+        W["name"] + ", %d receivers" % len(W["receivers"])
+
+
+def make_batch(sem, meshes, shots, path, nsteps):
     B = sem.Batch()
-    shots = []
-    for M, mw in zip(meshes, W["meshes"]):
-        for s in mw["shots"]:
-            k = B.add_shot(M, nsteps, mw["dt"])
-            B.add_moment_source(k, s["loc"], s["M"], s["amp"], s["f0"], s["t0"])
-            for r in mw["receivers"]:
-                B.add_receiver(k, r)
-            shots.append(k)
+    ids = []
+    for mi, dt, sources, receivers in shots:
+        k = B.add_shot(meshes[mi], nsteps, dt)
+        for s in sources:
+            if s["kind"] == "moment":
+                B.add_moment_source(k, s["loc"], s["M"], s["amp"], s["f0"], s["t0"])
+            else:
+                B.add_source(k, s["loc"], s["amp"], s["f0"], s["t0"], s["dir"])
+        for r in receivers:
+            B.add_receiver(k, r)
+        ids.append(k)
     B.prepare(path)
-    return B, shots
+    return B, ids
+
+
+def sample(workloads, name):
+    """Bounded accuracy / CPU-baseline sample of a workload."""
+    if name == "cfg4":
+        W = workloads.cfg4(sizes=(10,), shots_per_mesh=1, nsteps=250)
+        mw = W["meshes"][0]
+        return dict(dim=3, physics=1, cells=mw["cells"], coords=mw["coords"], par=mw["par"],
+                    dt=mw["dt"], nsteps=250, sources=mw["shots"], receivers=mw["receivers"],
+                    desc="cfg4 law on a 10^3 mesh, 1 moment-tensor shot, 250 steps, 3 receivers")
+    if name == "cfg3":
+        W = workloads.cfg3(n=16, nsteps=300, nrec_side=4)
+        desc = "cfg3 law on a 16^3 mesh, 300 steps, 16 receivers"
+    else:
+        W = workloads.cfg2(n=100, nsteps=300)
+        desc = "cfg2 law on 100^2 quads, 300 steps, 100 receivers"
+    return dict(dim=W["dim"], physics=W["physics"], cells=W["cells"], coords=W["coords"],
+                par=W["par"], dt=W["dt"], nsteps=300, sources=W["sources"],
+                receivers=W["receivers"], desc=desc)
+
+
+def profiled_traffic(name):
+    """HBM bytes per tile-kernel launch from the committed rocprofv3 summary
+    of this workload (FETCH_SIZE x fetch factor + WRITE_SIZE x write factor),
+    or None."""
+    p = ROOT / "profiles" / ("tile_kernel_%s_pmc.txt" % name)
+    if not p.exists():
+        return None
+    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
+    return float(m.group(1)) if m else None
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="cfg4", choices=("cfg4", "cfg3", "cfg2"))
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--plain-steps", type=int, default=20)
-    ap.add_argument("--sizes", default="32,48,64,96")
-    ap.add_argument("--shots", type=int, default=8, help="shots per mesh")
+    ap.add_argument("--sizes", default="32,48,64,96", help="cfg4 mesh sizes")
+    ap.add_argument("--shots", type=int, default=8, help="cfg4 shots per mesh")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the tile-shape autotuner")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=400.0)
     args = ap.parse_args()
@@ -118,19 +179,23 @@ def main():
 
     from salvus_amd import sem, workloads
 
-    sizes = tuple(int(x) for x in args.sizes.split(","))
-    W = workloads.cfg4(sizes=sizes, shots_per_mesh=args.shots)
-    t0 = time.perf_counter()
-    meshes = [sem.Mesh(3, 4, 1, mw["cells"], mw["coords"], mw["par"]) for mw in W["meshes"]]
-    log("meshes ready (%.1f s): %s" % (time.perf_counter() - t0,
-                                        ", ".join("%d^3 tiles %s" % (n, M.tile_shape)
-                                                  for n, M in zip(sizes, meshes))))
-    dofs = sum(M.c * M.G for M in meshes) * args.shots
-
-    # ---------------- batched path
     timing_steps = 10
     total = args.warmup + args.steps + timing_steps
-    B, shots = make_batch(sem, meshes, W, sem.PATH_FUSED, total)
+    sizes = tuple(int(x) for x in args.sizes.split(","))
+    t0 = time.perf_counter()
+    dim, physics, meshes, shots, desc = setup(sem, workloads, args.workload, total, sizes,
+                                              args.shots)
+    if not args.no_tune:  # tile shape (elements per workgroup) per mesh
+        for M in meshes:
+            M.autotune(reps=3)
+    log("meshes ready (%.1f s): %s" % (time.perf_counter() - t0,
+                                        ", ".join("E=%d tiles %s" % (M.E, M.tile_shape)
+                                                  for M in meshes)))
+    per_mesh = [sum(1 for s in shots if s[0] == i) for i in range(len(meshes))]
+    dofs = sum(M.c * M.G * k for M, k in zip(meshes, per_mesh))
+
+    # ---------------- batched path
+    B, ids = make_batch(sem, meshes, shots, sem.PATH_FUSED, total)
     cfg = B.config()
     log("batched path prepared: %s" % cfg)
     B.run(args.warmup)
@@ -138,7 +203,7 @@ def main():
     B.run(args.steps)
     fused_s = time.perf_counter() - t0
     value = dofs * args.steps / fused_s
-    log("batched: %.1f ms/step, %.3g DOF-updates/s" % (fused_s * 1e3 / args.steps, value))
+    log("batched: %.2f ms/step, %.3g DOF-updates/s" % (fused_s * 1e3 / args.steps, value))
 
     # kernel timing: eager launches bracketed by HIP events on the batch stream
     os.environ["SEM_TIME_KERNELS"] = "1"
@@ -147,14 +212,18 @@ def main():
     kms, nlaunch = B.kernel_time()
     launches_per_step = nlaunch / timing_steps
     avg_launch_s = kms / 1e3 / max(nlaunch, 1)
-    step_bytes = sum(element_bytes_per_step(M, args.shots, 3, 1) for M in meshes)
+    S = max(per_mesh)
+    groups = [(M, k) for M, k in zip(meshes, per_mesh)]
+    step_bytes = sum(element_bytes_per_step(M, k, dim, physics) for M, k in groups)
     per_launch_bytes = step_bytes / launches_per_step
     achieved = per_launch_bytes / avg_launch_s / 1e9
-    tile_launch_bytes = sum(tile_bytes_per_step(M, args.shots, 3, 1) for M in meshes) / launches_per_step
+    tile_launch_bytes = sum(tile_bytes_per_step(M, k, dim, physics) for M, k in groups) \
+        / launches_per_step
     tile_achieved = tile_launch_bytes / avg_launch_s / 1e9
     B.close()
     del B
-
+    traffic = profiled_traffic(args.workload)
+    shape = ",".join(map(str, meshes[-1].tile_shape or ()))
     out = {
         "metric": METRIC,
         "value": value,
@@ -167,17 +236,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded smooth Elastic3D models, seeded moment tensors)",
-        "config": {"workload": W["name"] + ", order 4, moment-tensor Ricker 15 Hz, "
-                               "3 receivers per shot",
-                   "meshes": ["%d^3" % n for n in sizes], "shots": len(shots),
+        "data": "synthetic (seeded models and sources)",
+        "config": {"workload": desc, "meshes": [int(M.E) for M in meshes], "shots": len(ids),
                    "dof_updates_per_step": dofs, "path": "batched (tile kernel)",
                    "tile_shapes": [list(M.tile_shape) if M.tile_shape else None for M in meshes],
                    "shots_per_group": cfg["shots_per_group"], "groups": cfg["groups"],
                    "graph_block_steps": cfg["block_steps"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_tile<3,4,elastic,%s>" % ",".join(map(str, meshes[0].tile_shape or ())),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_tile<%d,4,%s,%s>" % (dim, ("acoustic", "elastic")[physics], shape),
                      "algorithmic_bytes_per_launch": per_launch_bytes,
                      "bytes_model": "element formulation: per element n^d (4 + 12 c S) + 4 x "
                                     "(affine factors + vertex material); per node S (4 + 44 c)",
@@ -187,10 +254,12 @@ def main():
                      "launches_per_step": launches_per_step,
                      "kernel_ms_per_step": kms / timing_steps},
     }
+    if traffic is not None:
+        out["roofline"]["traffic_source"] = "profiles/tile_kernel_%s_pmc.txt" % args.workload
 
     # ---------------- plain path
     if not args.no_plain:
-        Bp, _ = make_batch(sem, meshes, W, sem.PATH_PLAIN, 2 + args.plain_steps)
+        Bp, _ = make_batch(sem, meshes, shots, sem.PATH_PLAIN, 2 + args.plain_steps)
         Bp.run(2)
         t0 = time.perf_counter()
         Bp.run(args.plain_steps)
@@ -201,7 +270,8 @@ def main():
         out["plain"] = {"value": plain_value, "steps": args.plain_steps,
                         "ms_per_step": plain_s * 1e3 / args.plain_steps}
         out["batched_vs_plain"] = value / plain_value
-        log("plain: %.1f ms/step, %.3g DOF-updates/s" % (plain_s * 1e3 / args.plain_steps, plain_value))
+        log("plain: %.2f ms/step, %.3g DOF-updates/s" % (plain_s * 1e3 / args.plain_steps,
+                                                         plain_value))
     for M in meshes:
         M.close()
 
@@ -209,35 +279,37 @@ def main():
     if not args.no_cpu:
         from oracle import pyoracle as po
         from tests.problems import rel_l2
-        n, nsteps = 10, 250
-        Ws = workloads.cfg4(sizes=(n,), shots_per_mesh=1, nsteps=nsteps)
-        mw = Ws["meshes"][0]
-        s = mw["shots"][0]
-        S = po.Sim(3, 4, 1, mw["cells"], mw["coords"], mw["par"])
-        S.add_moment_source(s["loc"], s["M"], s["amp"], s["f0"], s["t0"])
-        for r in mw["receivers"]:
-            S.add_receiver(r)
+        sm = sample(workloads, args.workload)
+        O = po.Sim(sm["dim"], 4, sm["physics"], sm["cells"], sm["coords"], sm["par"])
+        for s in sm["sources"]:
+            if s["kind"] == "moment":
+                O.add_moment_source(s["loc"], s["M"], s["amp"], s["f0"], s["t0"])
+            else:
+                O.add_source(s["loc"], s["amp"], s["f0"], s["t0"], s["dir"])
+        for r in sm["receivers"]:
+            O.add_receiver(r)
         t0 = time.perf_counter()
-        ref = S.run(mw["dt"], nsteps)
+        ref = O.run(sm["dt"], sm["nsteps"])
         cpu_s = time.perf_counter() - t0
-        Ms = sem.Mesh(3, 4, 1, mw["cells"], mw["coords"], mw["par"])
+        Ms = sem.Mesh(sm["dim"], 4, sm["physics"], sm["cells"], sm["coords"], sm["par"])
         errs = {}
         for name, path in (("batched", sem.PATH_FUSED), ("plain", sem.PATH_PLAIN)):
-            Bs, ks = make_batch(sem, [Ms], Ws, path, nsteps)
-            Bs.run(nsteps)
+            Bs, ks = make_batch(sem, [Ms], [(0, sm["dt"], sm["sources"], sm["receivers"])], path,
+                                sm["nsteps"])
+            Bs.run(sm["nsteps"])
             errs[name] = rel_l2(Bs.seismograms(ks[0]), ref)
             Bs.close()
-        Gs = (4 * n + 1) ** 3
+        dof_s = Ms.c * Ms.G * sm["nsteps"]
+        Ms.close()
         out["accuracy"] = {"receiver_rel_l2_batched": errs["batched"],
                            "receiver_rel_l2_plain": errs["plain"], "bar": 1e-5,
-                           "sample": "cfg4 law on a %d^3 mesh, 1 moment-tensor shot, %d steps, "
-                                     "3 receivers" % (n, nsteps)}
-        out["cpu_baseline"] = {"value": 3 * Gs * nsteps / cpu_s, "unit": "DOF-updates/s",
-                               "cores": 1, "kind": "port",
-                               "sample": "FP64 oracle, cfg4 law on a %d^3 mesh, %d steps (%.1f s)"
-                                         % (n, nsteps, cpu_s)}
+                           "sample": sm["desc"]}
+        out["cpu_baseline"] = {"value": dof_s / cpu_s, "unit": "DOF-updates/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "FP64 oracle, %s (%.1f s)" % (sm["desc"], cpu_s)}
+
     # ---------------- serving sample: continuous batching vs plain FIFO
-    if not args.no_serve:
+    if args.workload == "cfg4" and not args.no_serve:
         from salvus_amd import serve
         specs, reqs = serve.draw_stream(args.serve_requests, args.serve_rate, 5,
                                         models_per_category=2, size_scale=0.5, steps=(50, 300))

@@ -152,27 +152,54 @@ static bool pick_tile(int dim, int order, int physics, int *T) {
 
 static int upload_tile_words(sem_mesh *m) {
   const TileSet &ts = m->tiles;
-  if (!m->d_tile_words) SEM_TRY(dalloc(&m->d_tile_words, ts.words.size(), &m->bytes));
+  if (!m->d_tile_words) SEM_TRY(dalloc(&m->d_tile_words, ts.words.size(), &m->tile_bytes));
   HIPCHK(hipMemcpyAsync(m->d_tile_words, ts.words.data(), ts.words.size() * 4, hipMemcpyHostToDevice, m->stream));
   HIPCHK(hipStreamSynchronize(m->stream));
   return 0;
 }
 
-// Builds and uploads the tile path of an affine, logically Cartesian mesh.
-static int setup_tiles(sem_mesh *m) {
+void sem::free_tiles(sem_mesh *m) {
+  dfree(m->d_tile_info);
+  dfree(m->d_tile_words);
+  dfree(m->d_tgeo_e);
+  dfree(m->d_tpar_e);
+  dfree(m->d_mi_int);
+  m->bytes -= std::min(m->bytes, m->tile_bytes);
+  m->tile_bytes = 0;
+  m->tiles = TileSet();
+}
+
+// Builds and uploads the tile path of an affine, logically Cartesian mesh,
+// with the given tile shape or (shape null) the first of the preference list.
+int sem::setup_tiles(sem_mesh *m, const int *shape) {
+  free_tiles(m);
   const char *off = getenv("SEM_NO_TILES");
   if (!m->affine || (off && off[0] == '1')) return 0;
   int T[3];
-  if (!pick_tile(m->dim, m->order, m->physics, T)) return 0;
+  if (shape) {
+    T[0] = shape[0]; T[1] = shape[1]; T[2] = m->dim == 3 ? shape[2] : 1;
+    if (tile_kernel_info(m->dim, m->order, m->physics, T).lds_bytes == 0) {
+      set_error("tile shape not compiled for this order and physics");
+      return -1;
+    }
+  } else if (!pick_tile(m->dim, m->order, m->physics, T)) {
+    return 0;
+  }
   if (build_tiles(m, T[0], T[1], T[2], m->tiles) != 0) {
     m->tiles = TileSet();
     return 0;  // not eligible: the element kernel is used
   }
+  const int rc = upload_tiles(m);
+  m->bytes += m->tile_bytes;
+  return rc;
+}
+
+int sem::upload_tiles(sem_mesh *m) {
   TileSet &ts = m->tiles;
   const int dim = m->dim, nv = m->nv, npar = ref_npar(dim, m->physics), kp = kernel_npar(dim, m->physics);
   // padded tile element order (padding repeats element 0; never read)
   const int64_t E = (int64_t)ts.elem_order.size();
-  SEM_TRY(dalloc(&m->d_tile_info, ts.info.size(), &m->bytes));
+  SEM_TRY(dalloc(&m->d_tile_info, ts.info.size(), &m->tile_bytes));
   HIPCHK(hipMemcpyAsync(m->d_tile_info, ts.info.data(), ts.info.size() * sizeof(TileInfo), hipMemcpyHostToDevice, m->stream));
   SEM_TRY(upload_tile_words(m));
   // FP64 vertices in tile element order -> affine factors on the device
@@ -192,15 +219,15 @@ static int setup_tiles(sem_mesh *m) {
   double *d_tv = nullptr;
   SEM_TRY(dalloc(&d_tv, tv.size()));
   int rc = hip_check(hipMemcpyAsync(d_tv, tv.data(), tv.size() * 8, hipMemcpyHostToDevice, m->stream), "upload");
-  if (!rc) rc = dalloc(&m->d_tgeo_e, (size_t)E * affine_geo_count(dim, m->physics), &m->bytes);
+  if (!rc) rc = dalloc(&m->d_tgeo_e, (size_t)E * affine_geo_count(dim, m->physics), &m->tile_bytes);
   if (!rc) rc = launch_affine_geometry(dim, m->physics, E, d_tv, m->d_tgeo_e, m->stream);
-  if (!rc) rc = dalloc(&m->d_tpar_e, tp.size(), &m->bytes);
+  if (!rc) rc = dalloc(&m->d_tpar_e, tp.size(), &m->tile_bytes);
   if (!rc) rc = hip_check(hipMemcpyAsync(m->d_tpar_e, tp.data(), tp.size() * 4, hipMemcpyHostToDevice, m->stream), "upload");
   // inverse mass in the internal numbering
   int32_t *d_to = nullptr;
   if (!rc) rc = dalloc(&d_to, (size_t)m->G);
   if (!rc) rc = hip_check(hipMemcpyAsync(d_to, ts.to_internal.data(), m->G * 4, hipMemcpyHostToDevice, m->stream), "upload");
-  if (!rc) rc = dalloc(&m->d_mi_int, (size_t)m->G, &m->bytes);
+  if (!rc) rc = dalloc(&m->d_mi_int, (size_t)m->G, &m->tile_bytes);
   if (!rc) rc = launch_permute_f32(m->G, d_to, m->d_mi, m->d_mi_int, m->stream);
   if (!rc) rc = hip_check(hipStreamSynchronize(m->stream), "sync");
   dfree(d_tv);
@@ -398,7 +425,7 @@ int sem_mesh_create(sem_mesh **out, int device, int dim, int order, int physics,
     if (rc) return fail(rc);
   }
   if (!affine && (rc = ensure_pointwise(m))) return fail(rc);
-  if ((rc = setup_tiles(m))) return fail(rc);
+  if ((rc = setup_tiles(m, nullptr))) return fail(rc);
   *out = m;
   return 0;
 }

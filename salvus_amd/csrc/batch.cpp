@@ -521,6 +521,55 @@ static double time_shape(sem_batch *b, int reps) {
   return ms;
 }
 
+// Full batched-path steps (state advances; step counters do not), timed with
+// HIP events after one warm step.
+static double time_steps(sem_batch *b, int reps) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1;
+  if (enqueue_fused_step(b, 0, 0)) return -1;
+  (void)hipEventRecord(e0, b->stream);
+  for (int r = 0; r < reps; ++r)
+    if (enqueue_fused_step(b, 0, 0)) return -1;
+  (void)hipEventRecord(e1, b->stream);
+  (void)hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms;
+}
+
+using Bucket = std::map<std::tuple<sem_mesh *, int64_t, double>, std::vector<int>>;
+static Bucket buckets_of(const sem_batch *b) {
+  Bucket bk;
+  for (int i = 0; i < (int)b->shots.size(); ++i)
+    bk[std::make_tuple(b->shots[i].mesh, b->shots[i].nsteps, b->shots[i].dt)].push_back(i);
+  return bk;
+}
+
+static int largest_bucket(const sem_batch *b) {
+  size_t mx = 0;
+  for (auto &kv : buckets_of(b)) mx = std::max(mx, kv.second.size());
+  return (int)mx;
+}
+
+// Groups of up to max_s shots sharing mesh, nsteps and dt.
+static void form_groups(sem_batch *b, int max_s) {
+  b->groups.clear();
+  for (auto &kv : buckets_of(b)) {
+    auto &list = kv.second;
+    for (size_t k = 0; k < list.size(); k += max_s) {
+      Group g;
+      g.mesh = std::get<0>(kv.first);
+      g.nsteps = std::get<1>(kv.first);
+      g.dt = std::get<2>(kv.first);
+      for (size_t j = k; j < std::min(list.size(), k + (size_t)max_s); ++j) g.shots.push_back(list[j]);
+      g.S = (int)g.shots.size();
+      b->groups.push_back(g);
+    }
+  }
+}
+
 static Shot *get_shot(sem_batch *b, int shot) {
   if (shot < 0 || shot >= (int)b->shots.size()) {
     set_error("shot index out of range");
@@ -667,26 +716,37 @@ int sem_batch_prepare(sem_batch *b, int path, int epb, int spg) {
   if (path != SEM_PATH_PLAIN && path != SEM_PATH_FUSED) { set_error("path"); return -1; }
   HIPCHK(hipSetDevice(b->device));
   b->path = path;
-  // ---- grouping: shots sharing mesh, nsteps and dt (batched path only)
-  const int max_s = path == SEM_PATH_PLAIN ? 1 : (spg > 0 ? spg : 8);
-  std::map<std::tuple<sem_mesh *, int64_t, double>, std::vector<int>> buckets;
-  for (int i = 0; i < (int)b->shots.size(); ++i)
-    buckets[std::make_tuple(b->shots[i].mesh, b->shots[i].nsteps, b->shots[i].dt)].push_back(i);
-  for (auto &kv : buckets) {
-    auto &list = kv.second;
-    for (size_t k = 0; k < list.size(); k += max_s) {
-      Group g;
-      g.mesh = std::get<0>(kv.first);
-      g.nsteps = std::get<1>(kv.first);
-      g.dt = std::get<2>(kv.first);
-      for (size_t j = k; j < std::min(list.size(), k + (size_t)max_s); ++j) g.shots.push_back(list[j]);
-      g.S = (int)g.shots.size();
-      b->groups.push_back(g);
+  auto fail = [&](int r) { batch_release(b); return r; };
+  int rc = 0;
+  // ---- shots per group: given, 1 on the plain path, or autotuned: every
+  // power of two up to the largest bucket is set up and timed over full
+  // steps (the state is rebuilt from the shot specifications afterwards)
+  int max_s = path == SEM_PATH_PLAIN ? 1 : spg;
+  if (max_s <= 0) {
+    const int biggest = largest_bucket(b);
+    max_s = std::min(biggest, 8);
+    if (biggest > 1) {
+      double best = 1e300;
+      for (int cand = 1; cand <= std::min(biggest, 8); cand *= 2) {
+        form_groups(b, cand);
+        b->h_groups.resize(b->groups.size());
+        for (int gi = 0; gi < (int)b->groups.size() && !rc; ++gi) rc = setup_group(b, gi);
+        if (!rc) rc = finalize_groups(b, epb);
+        const double t = rc ? -1 : time_steps(b, 2);
+        batch_release(b);
+        if (rc) return rc;
+        if (t < 0) return fail(-2);
+        if (t < best * 0.98) {
+          best = t;
+          max_s = cand;
+        }
+      }
+      b->path = path;
     }
   }
-  auto fail = [&](int r) { batch_release(b); return r; };
+  // ---- grouping: shots sharing mesh, nsteps and dt
+  form_groups(b, max_s);
   b->h_groups.resize(b->groups.size());
-  int rc = 0;
   for (int gi = 0; gi < (int)b->groups.size(); ++gi)
     if ((rc = setup_group(b, gi))) return fail(rc);
   if ((rc = finalize_groups(b, epb))) return fail(rc);
@@ -806,6 +866,36 @@ int sem_batch_config(const sem_batch *b, int64_t *cfg) {
 int sem_batch_kernel_time(const sem_batch *b, double *ms, int64_t *launches) {
   *ms = b->kernel_ms;
   *launches = b->kernel_launches;
+  return 0;
+}
+
+int sem_mesh_autotune(sem_mesh *m, int reps, int64_t *shape) {
+  *shape = 0;
+  HIPCHK(hipSetDevice(m->device));
+  if (!m->tiles.valid) return 0;  // element kernel: nothing to choose per mesh
+  int shapes[3 * 16];
+  const int ns = tile_kernel_shapes(m->dim, m->order, m->physics, shapes, 16);
+  double best = 1e300;
+  int bi = -1;
+  for (int i = 0; i < ns; ++i) {
+    SEM_TRY(setup_tiles(m, &shapes[3 * i]));
+    if (!m->tiles.valid) continue;
+    sem_batch *b = nullptr;
+    SEM_TRY(sem_batch_create(&b, m->device));
+    int k = 0;
+    int rc = sem_batch_add_shot(b, m, reps + 2, 1.0, &k);
+    if (!rc) rc = sem_batch_prepare(b, SEM_PATH_FUSED, default_epb(m->nd), 1);
+    const double t = rc ? -1 : time_steps(b, std::max(reps, 1));
+    sem_batch_destroy(b);
+    if (rc) return rc;
+    if (t < 0) { set_error("autotune timing failed"); return -2; }
+    if (t < best * 0.98) {
+      best = t;
+      bi = i;
+    }
+  }
+  SEM_TRY(setup_tiles(m, bi >= 0 ? &shapes[3 * bi] : nullptr));
+  if (m->tiles.valid) *shape = m->tiles.T[0] * 10000 + m->tiles.T[1] * 100 + m->tiles.T[2];
   return 0;
 }
 

@@ -121,6 +121,11 @@ int ensure_pointwise(sem_mesh *m);
 int locate(const sem_mesh *m, const double *pt, int64_t *elem, double *ref);
 int source_load(const sem_mesh *m, const SrcSpec &s, int64_t *elem, std::vector<double> &load);
 double ricker(double t, double amp, double f0, double t0);  // Ricker.cpp:27-33
+// Tile path of a mesh with the given shape (null: preference list); frees
+// any previous tiling first.
+int setup_tiles(sem_mesh *m, const int *shape);
+int upload_tiles(sem_mesh *m);
+void free_tiles(sem_mesh *m);
 // batch.cpp
 // Position of DMPlex DOF entry i (= comp*G + g) in a group's field arrays.
 int64_t field_pos(const Group &g, int64_t i);

@@ -248,6 +248,8 @@ struct TileKernelInfo {
   int threads;
 };
 TileKernelInfo tile_kernel_info(int dim, int order, int physics, const int *T);
+// Compiled tile shapes for (dim, order, physics): up to max triples; returns the count.
+int tile_kernel_shapes(int dim, int order, int physics, int *shapes, int max);
 // Readies the kernel of this shape (LDS limit) and returns its workgroups per CU.
 int tile_kernel_prepare(int dim, int order, int physics, const int *T, int *per_cu);
 // One workgroup per tile: grid = a.nitems.
@@ -285,7 +287,8 @@ struct sem_mesh {
   sem::TileInfo *d_tile_info = nullptr;
   uint32_t *d_tile_words = nullptr;
   float *d_tgeo_e = nullptr, *d_tpar_e = nullptr, *d_mi_int = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0;       // device bytes held (tile path included)
+  size_t tile_bytes = 0;  // of which the tile path
 };
 
 namespace sem {

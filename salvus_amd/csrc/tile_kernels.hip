@@ -125,9 +125,14 @@ struct TS {
   // LDS layout (floats): u lattice | words | lateral fluxes of one chunk |
   // element-local results | element data | tile source list (4 ints each)
   static constexpr int O_SW = C * NTL;
+  // With a single chunk the element-local results overwrite the lateral
+  // fluxes once every column has read them (one more barrier, less LDS).
+  static constexpr bool ALIAS = NCH == 1;
+  // register budget: 4 waves per SIMD (<= 128 VGPRs)
+  static constexpr int MINW = 4;
   static constexpr int O_SF = O_SW + NTL;
-  static constexpr int O_SK = O_SF + EC * C * NL * ND;
-  static constexpr int O_SE = O_SK + E * C * ND;
+  static constexpr int O_SK = ALIAS ? O_SF : O_SF + EC * C * NL * ND;
+  static constexpr int O_SE = ALIAS ? O_SF + EC * C * NL * ND : O_SK + E * C * ND;
   static constexpr int O_SS = O_SE + E * NED;
   static constexpr int LDS = O_SS + 4 * kTileMaxSrc + 1;
 };
@@ -154,7 +159,8 @@ __device__ __forceinline__ float p1w(int v, float a, float b, float t) {
 }
 
 template <int DIM, int N, int PHYS, int TX, int TY, int TZ>
-__global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT)) k_tile(FusedArgs A) {
+__global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
+    __attribute__((amdgpu_waves_per_eu(TS<DIM, N, PHYS, TX, TY, TZ>::MINW))) k_tile(FusedArgs A) {
   using T = TS<DIM, N, PHYS, TX, TY, TZ>;
   constexpr int n = T::n, C = T::C, ND = T::ND, NL = T::NL;
   extern __shared__ float lds[];
@@ -421,8 +427,8 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT)) k_tile(Fus
         }
       }
       __syncthreads();
+      float f[C][n];
       if (act) {
-        float f[C][n];
 #pragma unroll
         for (int k = 0; k < n; ++k) {
           const int p = pl + T::ES * k;
@@ -452,6 +458,9 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT)) k_tile(Fus
 #pragma unroll
             for (int k = 0; k < n; ++k) f[c][k] = fmaf(ld[c * ND + pl + T::ES * k], R, f[c][k]);
         }
+      }
+      if constexpr (T::ALIAS) __syncthreads();  // lateral fluxes read before sK overwrites them
+      if (act) {
 #pragma unroll
         for (int c = 0; c < C; ++c)
 #pragma unroll
@@ -618,6 +627,23 @@ TileKernelInfo tile_kernel_info(int dim, int order, int physics, const int *T) {
   SEM_TILE_KERNELS(TKI)
 #undef TKI
   return r;
+}
+
+int tile_kernel_shapes(int dim, int order, int physics, int *shapes, int max) {
+  int k = 0;
+#define TSH(D, N, X, Y, Z)                                                                  \
+  if (dim == D && order == N && k < max) {                                                  \
+    const int T[3] = {X, Y, D == 3 ? Z : 1};                                                \
+    if (tile_kernel_info(dim, order, physics, T).lds_bytes <= 160 * 1024) {                 \
+      shapes[3 * k] = T[0];                                                                 \
+      shapes[3 * k + 1] = T[1];                                                             \
+      shapes[3 * k + 2] = T[2];                                                             \
+      ++k;                                                                                  \
+    }                                                                                       \
+  }
+  SEM_TILE_KERNELS(TSH)
+#undef TSH
+  return k;
 }
 
 // Sets the dynamic-LDS limit of a kernel above 64 KiB and returns how many

@@ -38,6 +38,7 @@ EXPORTS = (
     "sem_shot_get_seismograms", "sem_batch_config", "sem_batch_kernel_time",
     "sem_reference_step_count", "sem_server_create", "sem_server_destroy", "sem_server_clock",
     "sem_server_submit", "sem_server_pump", "sem_server_request", "sem_server_result",
+    "sem_mesh_autotune",
 )
 SOURCE_DOUBLES = 19
 
@@ -101,6 +102,7 @@ def lib():
         L.sem_server_pump.argtypes = [_VP, _D]
         L.sem_server_request.argtypes = [_VP, C.c_int, _D]
         L.sem_server_result.argtypes = [_VP, C.c_int, _F]
+        L.sem_mesh_autotune.argtypes = [_VP, C.c_int, _I64]
         _lib = L
     return _lib
 
@@ -170,6 +172,9 @@ class Mesh:
         self._h = _VP()
         _check(lib().sem_mesh_create(C.byref(self._h), device, dim, order, physics, self.E, self.V,
                                      _i(cells), _d(coords), _d(par), quirks))
+        self._read_info()
+
+    def _read_info(self):
         info = np.zeros(16, dtype=np.int64)
         lib().sem_mesh_info(self._h, _l(info))
         self.G, self.ncolours, self.c, self.nd = (int(x) for x in info[1:5])
@@ -181,6 +186,14 @@ class Mesh:
         self.tile_slots, self.tile_valid_slots = int(info[10]), int(info[11])
         self.tile_acc_reads, self.tile_acc_writes = int(info[12]), int(info[13])
         self.tile_elems = int(info[14])
+
+    def autotune(self, reps=3):
+        """Times every compiled tile shape over `reps` steps and keeps the
+        fastest (call before the mesh joins a batch); returns the shape."""
+        v = np.zeros(1, dtype=np.int64)
+        _check(lib().sem_mesh_autotune(self._h, reps, _l(v)))
+        self._read_info()
+        return self.tile_shape
 
     def close(self):
         if getattr(self, "_h", None):
