@@ -109,7 +109,7 @@ def setup(sem, workloads, name, nsteps, sizes, shots_per_mesh):
         W["name"] + ", %d receivers" % len(W["receivers"])
 
 
-def make_batch(sem, meshes, shots, path, nsteps):
+def make_batch(sem, meshes, shots, path, nsteps, spg=0):
     B = sem.Batch()
     ids = []
     for mi, dt, sources, receivers in shots:
@@ -122,7 +122,7 @@ def make_batch(sem, meshes, shots, path, nsteps):
         for r in receivers:
             B.add_receiver(k, r)
         ids.append(k)
-    B.prepare(path)
+    B.prepare(path, 0, spg)
     return B, ids
 
 
@@ -169,6 +169,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile-shape autotuner")
+    ap.add_argument("--spg", type=int, default=0, help="shots per group (0: autotuned)")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=400.0)
     args = ap.parse_args()
@@ -195,7 +196,7 @@ def main():
     dofs = sum(M.c * M.G * k for M, k in zip(meshes, per_mesh))
 
     # ---------------- batched path
-    B, ids = make_batch(sem, meshes, shots, sem.PATH_FUSED, total)
+    B, ids = make_batch(sem, meshes, shots, sem.PATH_FUSED, total, args.spg)
     cfg = B.config()
     log("batched path prepared: %s" % cfg)
     B.run(args.warmup)

@@ -139,13 +139,13 @@ static bool pick_tile(int dim, int order, int physics, int *T) {
   }
   static const int pref2[][3] = {{16, 8, 1}, {8, 8, 1}};
   static const int pref3a[][3] = {{4, 4, 2}, {4, 2, 2}};
-  static const int pref3e[][3] = {{2, 2, 2}, {4, 2, 2}};
+  static const int pref3e[][3] = {{4, 2, 2}, {2, 2, 2}};
   const int(*list)[3] = dim == 2 ? pref2 : (physics == SEM_ACOUSTIC ? pref3a : pref3e);
   const int cnt = 2;
   for (int i = 0; i < cnt; ++i) {
     T[0] = list[i][0]; T[1] = list[i][1]; T[2] = list[i][2];
     const TileKernelInfo k = tile_kernel_info(dim, order, physics, T);
-    if (k.lds_bytes > 0 && k.lds_bytes <= 64 * 1024) return true;
+    if (k.lds_bytes > 0 && k.lds_bytes <= 160 * 1024) return true;
   }
   return false;
 }

@@ -11,8 +11,9 @@
 //
 // Tiles are coloured by lattice parity (2^dim colours; tiles of one colour
 // share no node).  Nodes are renumbered tile-major: each tile's new nodes in
-// slot order, its inside first and then one run per +x / +y (/ +z) face, so
-// that a tile's own nodes and each face its neighbours share are contiguous.
+// slot order, its inside first and then one run per upper face, edge and
+// corner, so that a tile's own nodes and each piece its neighbours share are
+// contiguous.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -98,20 +99,21 @@ int build_tiles(sem_mesh *m, int Tx, int Ty, int Tz, TileSet &ts) {
               }
             }
       }
-  // ---- internal numbering: tile-major; a tile's new nodes come in runs --
-  // inside, then its +x face, +y face (and +z face) -- so that every face a
-  // neighbouring tile reads or accumulates is one contiguous range
+  // ---- internal numbering: tile-major; a tile's new nodes come in runs by
+  // which of its upper faces they lie on (none, +x, +y, +x+y edge, +z, ...),
+  // so that every face, edge or corner a neighbouring tile reads or
+  // accumulates is one contiguous range
   ts.to_internal.assign(m->G, -1);
   int64_t next = 0;
   for (int64_t t = 0; t < ntiles; ++t) {
     const TileInfo &ti = ts.info[t];
     const int fx = ti.ex * N, fy = ti.ey * N, fz = ti.ez * N;  // last lattice index
-    for (int cat = 0; cat < (dim == 3 ? 4 : 3); ++cat)
+    for (int cat = 0; cat < (1 << dim); ++cat)
       for (int64_t s = 0; s < NTL; ++s) {
         const int32_t g = slot_node[t * NTL + s];
         if (g < 0 || ts.to_internal[g] >= 0) continue;
         const int x = (int)(s % LX), y = (int)((s / LX) % LY), z = (int)(s / ((int64_t)LX * LY));
-        const int c = x == fx ? 1 : (y == fy ? 2 : (dim == 3 && z == fz ? 3 : 0));
+        const int c = (x == fx) | ((y == fy) << 1) | ((dim == 3 && z == fz) << 2);
         if (c == cat) ts.to_internal[g] = (int32_t)next++;
       }
   }
