@@ -1,0 +1,60 @@
+"""Host-side logic of the serving driver and the benchmark's byte models
+(no GPU): the request stream is reproducible and respects the configuration-5
+ranges; source records have the C ABI layout; the roofline byte formulas
+count what they state on a hand-checked case."""
+import types
+
+import numpy as np
+
+import bench
+from salvus_amd import sem, serve
+
+
+def test_stream_reproducible_and_in_range():
+    specs, reqs = serve.draw_stream(400, 20.0, seed=5, models_per_category=4)
+    specs2, reqs2 = serve.draw_stream(400, 20.0, seed=5, models_per_category=4)
+    assert specs == specs2
+    assert [r["arrival"] for r in reqs] == [r["arrival"] for r in reqs2]
+    arr = np.array([r["arrival"] for r in reqs])
+    assert np.all(np.diff(arr) > 0)
+    assert abs(len(reqs) / arr[-1] - 20.0) < 4.0  # Poisson rate
+    assert all(500 <= r["nsteps"] <= 3000 for r in reqs)
+    ranges = {0: (100, 800), 1: (200, 1000), 2: (32, 96)}
+    for s in specs:
+        lo, hi = ranges[s["category"]]
+        assert lo <= s["n"] <= hi
+        assert s["dim"] == (3 if s["category"] == 2 else 2)
+    cats = np.bincount([specs[r["model"]]["category"] for r in reqs], minlength=3) / len(reqs)
+    assert np.allclose(cats, [0.4, 0.3, 0.3], atol=0.08)
+    assert {s["physics"] for s in specs if s["category"] == 2} == {0, 1}
+
+
+def test_size_scale():
+    specs, _ = serve.draw_stream(10, 5.0, seed=1, models_per_category=3, size_scale=0.5)
+    for s in specs:
+        lo, hi = {0: (100, 800), 1: (200, 1000), 2: (32, 96)}[s["category"]]
+        assert 0.5 * lo - 1 <= s["n"] <= 0.5 * hi + 1
+
+
+def test_source_record_layout():
+    r = sem.source_record(dict(kind="force", loc=(1.0, 2.0), amp=3.0, f0=4.0, t0=5.0,
+                               dir=(0.0, 1.0)))
+    assert r.shape == (sem.SOURCE_DOUBLES,)
+    assert list(r[:10]) == [0, 1, 2, 0, 3, 4, 5, 0, 1, 0]
+    M = np.arange(9.0).reshape(3, 3)
+    r = sem.source_record(dict(kind="moment", loc=(1.0, 2.0, 3.0), M=M, amp=1.0, f0=2.0,
+                               t0=0.5))
+    assert r[0] == 1 and list(r[1:4]) == [1, 2, 3]
+    assert list(r[10:19]) == list(M.ravel())
+
+
+def test_byte_models_on_a_hand_case():
+    # one 2-D acoustic element of order 1 (4 nodes), 1 tile of 1x1 elements
+    M = types.SimpleNamespace(c=1, nd=4, E=1, G=4, ntiles=1, tile_elems=1, tile_slots=4,
+                              tile_valid_slots=4, tile_acc_reads=0, tile_acc_writes=0)
+    # element formulation: 4 nodes x (4 + 12) + 4 x (3 factors + 4 vertex vp) + 4 x (4 + 44)
+    assert bench.element_bytes_per_step(M, 1, 2, 0) == 4 * 16 + 4 * 7 + 4 * 48
+    # tile algorithm: 4 words + 4 x 7 element floats + (4 u + 4 x (4 + 36))
+    assert bench.tile_bytes_per_step(M, 1, 2, 0) == 16 + 28 + 16 + 4 * 40
+    # S shots scale the per-shot parts only
+    assert bench.tile_bytes_per_step(M, 3, 2, 0) == 16 + 28 + 3 * (16 + 4 * 40)
