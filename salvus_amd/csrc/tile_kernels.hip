@@ -134,7 +134,8 @@ struct TS {
   static constexpr int O_SK = ALIAS ? O_SF : O_SF + EC * C * NL * ND;
   static constexpr int O_SE = ALIAS ? O_SF + EC * C * NL * ND : O_SK + E * C * ND;
   static constexpr int O_SS = O_SE + E * NED;
-  static constexpr int LDS = O_SS + 4 * kTileMaxSrc + 1;
+  static constexpr int O_SD = O_SS + 4 * kTileMaxSrc + 1;
+  static constexpr int LDS = O_SD + n * n;
 };
 
 __device__ __forceinline__ void kahan(float &s, float &c, float x) {
@@ -170,6 +171,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
   float *sK = lds + T::O_SK;
   float *sE = lds + T::O_SE;
   int *sS = reinterpret_cast<int *>(lds + T::O_SS);  // {element, shot, stf_off, load_off} x k, count
+  float *sD = lds + T::O_SD;  // GLL derivative matrix
 
   const int bid = blockIdx.x, nb = gridDim.x;
   int item = bid;
@@ -183,7 +185,10 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
   const int tid = threadIdx.x;
   const TileTables &TT = t_tab[N];
 
-  // ---- once per tile: node words, element data, sources inside the tile
+  // ---- once per tile: node words, element data, sources inside the tile,
+  // and the lattice of shot 0 (every later shot's lattice is loaded during
+  // the previous shot's face passes).  Outside the column phase each thread
+  // writes and reads only its own slots l = tid + r BT of su.
   {
     const uint32_t *__restrict__ words = G.tile_words + ti.node_off;
     uint32_t wv[T::NIT];
@@ -192,11 +197,15 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
       const int l = tid + r * T::BT;
       wv[r] = l < T::NTL ? words[l] : kSlotEmpty;
     }
+    float uv[T::NIT][C];
 #pragma unroll
     for (int r = 0; r < T::NIT; ++r) {
-      const int l = tid + r * T::BT;
-      if (l < T::NTL) sw[l] = wv[r];
+      const bool ok = wv[r] != kSlotEmpty;
+      const uint32_t g = ok ? (wv[r] & kNodeMask) : 0u;
+#pragma unroll
+      for (int c = 0; c < C; ++c) uv[r][c] = ok ? G.u[c * (uint32_t)G.G + g] : 0.f;
     }
+    for (int i = tid; i < n * n; i += T::BT) sD[i] = TT.D[i];
     const float *__restrict__ tg = G.tgeo_e + (int64_t)ti.elem_off * T::NG;
     const float *__restrict__ tp = G.tpar_e + (int64_t)ti.elem_off * (T::NV * T::KP);
     for (int i = tid; i < T::E * T::NED; i += T::BT) {
@@ -227,6 +236,15 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
       }
       if (tid == 0) sS[4 * kTileMaxSrc] = cnt < kTileMaxSrc ? cnt : kTileMaxSrc;
     }
+#pragma unroll
+    for (int r = 0; r < T::NIT; ++r) {
+      const int l = tid + r * T::BT;
+      if (l < T::NTL) {
+        sw[l] = wv[r];
+#pragma unroll
+        for (int c = 0; c < C; ++c) su[c * T::NTL + l] = uv[r][c];
+      }
+    }
   }
 
   // ---- the column this thread owns in every chunk
@@ -235,14 +253,6 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
   const int ci = col % n;
   const int cj = DIM == 3 ? col / n : 0;
   const bool lane = ec < T::EC;
-  float Dx[n], Dy[n], Dcx[n], Dcy[n];
-#pragma unroll
-  for (int m = 0; m < n; ++m) {
-    Dx[m] = TT.D[ci * n + m];
-    Dcx[m] = TT.D[m * n + ci];
-    Dy[m] = DIM == 3 ? TT.D[cj * n + m] : 0.f;
-    Dcy[m] = DIM == 3 ? TT.D[m * n + cj] : 0.f;
-  }
   const float wl = DIM == 3 ? TT.w[ci] * TT.w[cj] : TT.w[ci];
   const float xa = TT.xi[ci], xb = DIM == 3 ? TT.xi[cj] : 0.f;
   const int pl = ci + (DIM == 3 ? n * cj : 0);  // element-local index of the column's node 0
@@ -257,33 +267,10 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
   const uint32_t GG = (uint32_t)G.G, Sg = (uint32_t)G.S;
   const bool q3 = (C == 3) && (G.quirks & SEM_QUIRK_NEWMARK3D);
 
-  for (uint32_t s = 0; s < Sg; ++s) {
-    __syncthreads();  // tile data staged / previous shot done with su and sK
-    // ---- gather
-    {
-      float uv[T::NIT][C];
-      bool ok[T::NIT];
-#pragma unroll
-      for (int r = 0; r < T::NIT; ++r) {
-        const int l = tid + r * T::BT;
-        const uint32_t w = l < T::NTL ? sw[l] : kSlotEmpty;
-        ok[r] = w != kSlotEmpty;
-        const uint32_t g = ok[r] ? (w & kNodeMask) : 0u;
-#pragma unroll
-        for (int c = 0; c < C; ++c) uv[r][c] = ok[r] ? gu[(s * C + c) * GG + g] : 0.f;
-      }
-#pragma unroll
-      for (int r = 0; r < T::NIT; ++r) {
-        const int l = tid + r * T::BT;
-        if (ok[r]) {
-#pragma unroll
-          for (int c = 0; c < C; ++c) su[c * T::NTL + l] = uv[r][c];
-        }
-      }
-    }
-    __syncthreads();
-    const int nts = sS[4 * kTileMaxSrc];
+  __syncthreads();  // tile data and the first shot's lattice staged
+  const int nts = sS[4 * kTileMaxSrc];
 
+  for (uint32_t s = 0; s < Sg; ++s) {
     // ---- columns, one chunk of elements at a time
 #pragma unroll 1
     for (int ch = 0; ch < T::NCH; ++ch) {
@@ -301,6 +288,12 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
       float Fz[C][n];
       if (act) {
         const float *ed = sE + e * T::NED;
+        float Dx[n], Dy[n];  // rows ci and cj of D, from LDS
+#pragma unroll
+        for (int m = 0; m < n; ++m) {
+          Dx[m] = sD[ci * n + m];
+          Dy[m] = DIM == 3 ? sD[cj * n + m] : 0.f;
+        }
         float uc[C][n];
 #pragma unroll
         for (int c = 0; c < C; ++c)
@@ -429,6 +422,12 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
       __syncthreads();
       float f[C][n];
       if (act) {
+        float Dcx[n], Dcy[n];  // columns ci and cj of D
+#pragma unroll
+        for (int m = 0; m < n; ++m) {
+          Dcx[m] = sD[m * n + ci];
+          Dcy[m] = DIM == 3 ? sD[m * n + cj] : 0.f;
+        }
 #pragma unroll
         for (int k = 0; k < n; ++k) {
           const int p = pl + T::ES * k;
@@ -469,6 +468,45 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
     }
     __syncthreads();
 
+    // ---- finish inputs of the first KF slots and the next shot's lattice
+    // are loaded here so that their latency overlaps the face passes
+    struct SlotIn {
+      uint32_t w;
+      float mi, prev[C], ap[C], v[C], kv[C], ku[C];
+    };
+    auto load_slot = [&](int r, SlotIn &in) {
+      const int l = tid + r * T::BT;
+      in.w = (r < T::NIT && l < T::NTL) ? sw[l] : kSlotEmpty;
+      const bool valid = in.w != kSlotEmpty;
+      const uint32_t g = valid ? (in.w & kNodeMask) : 0u;
+      const bool first = (in.w & kFlagFirst) != 0;
+      const bool fin = valid && (in.w & kFlagLast) && !A.ku_only;
+      in.mi = fin ? gmi[g] : 0.f;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const uint32_t idx = (s * C + c) * GG + g;
+        in.prev[c] = (valid && !first) ? gacc[idx] : 0.f;
+        in.ap[c] = fin ? gap[idx] : 0.f;
+        in.v[c] = fin ? gv[idx] : 0.f;
+        in.kv[c] = fin ? gkv[idx] : 0.f;
+        in.ku[c] = fin ? gku[idx] : 0.f;
+      }
+    };
+    constexpr int KF = T::NIT < 64 / (5 * C + 2) ? T::NIT : 64 / (5 * C + 2);
+    SlotIn pre[KF];
+#pragma unroll
+    for (int r = 0; r < KF; ++r) load_slot(r, pre[r]);
+    const bool more = s + 1 < Sg;
+    float nu[T::NIT][C];
+#pragma unroll
+    for (int r = 0; r < T::NIT; ++r) {
+      const int l = tid + r * T::BT;
+      const uint32_t w = (more && l < T::NTL) ? sw[l] : kSlotEmpty;
+      const uint32_t g = w != kSlotEmpty ? (w & kNodeMask) : 0u;
+#pragma unroll
+      for (int c = 0; c < C; ++c) nu[r][c] = w != kSlotEmpty ? gu[((s + 1) * C + c) * GG + g] : 0.f;
+    }
+
     // ---- shared faces: x, then y, then z (fixed order)
 #pragma unroll
     for (int ax = 0; ax < DIM; ++ax) {
@@ -499,69 +537,63 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
       __syncthreads();
     }
 
-    // ---- finish every lattice slot
+    // ---- finish every lattice slot: total of the owner element's face-summed
+    // result, accumulation over colours, Newmark update at the last colour
+    auto finish_slot = [&](int r, const SlotIn &in) {
+      if (in.w == kSlotEmpty) return;
+      const int l = tid + r * T::BT;
+      const int x = l % T::LX, y = (l / T::LX) % T::LY, z = DIM == 3 ? l / T::LXY : 0;
+      const int ea = min(x / N, ti.ex - 1), eb = min(y / N, ti.ey - 1);
+      const int ez = DIM == 3 ? min(z / N, ti.ez - 1) : 0;
+      const int e = ea + TX * (eb + TY * ez);
+      const int p = (x - ea * N) + n * ((y - eb * N) + n * (z - ez * N));
+      const uint32_t g = in.w & kNodeMask;
+      const bool first = (in.w & kFlagFirst) != 0;
+      const bool fin = (in.w & kFlagLast) && !A.ku_only;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const uint32_t idx = (s * C + c) * GG + g;
+        const float tot = sK[(e * C + c) * ND + p];
+        const float t = first ? tot : in.prev[c] + tot;
+        if (!fin) {
+          gacc[idx] = t;
+          continue;
+        }
+        if (q3 && c == 2) continue;
+        const float a = in.mi * t;
+        float ap = in.ap[c], v = in.v[c], kv = in.kv[c];
+        float u = su[c * T::NTL + l], ku = in.ku[c];
+        const int reps = (q3 && c == 0) ? 2 : 1;
+        for (int rep = 0; rep < reps; ++rep) {
+          kahan(v, kv, G.acl * (a + ap));
+          kahan(u, ku, fmaf(G.dt, v, G.dsp * a));
+          ap = a;
+        }
+        gu[idx] = u; gv[idx] = v; gap[idx] = a; gku[idx] = ku; gkv[idx] = kv;
+      }
+    };
+#pragma unroll
+    for (int r = 0; r < KF; ++r) finish_slot(r, pre[r]);
     constexpr int KB = C == 1 ? 4 : 2;
 #pragma unroll 1
-    for (int r0 = 0; r0 < T::NIT; r0 += KB) {
-      uint32_t wk[KB];
-      float tot[KB][C], prev[KB][C], vv[KB][C], kvv[KB][C], kuu[KB][C], app[KB][C], mi[KB];
+    for (int r0 = KF; r0 < T::NIT; r0 += KB) {
+      SlotIn in[KB];
 #pragma unroll
-      for (int r = 0; r < KB; ++r) {
-        const int l = tid + (r0 + r) * T::BT;
-        wk[r] = (r0 + r < T::NIT && l < T::NTL) ? sw[l] : kSlotEmpty;
-        const bool valid = wk[r] != kSlotEmpty;
-        int e = 0, p = 0;
-        if (valid) {
-          const int x = l % T::LX, y = (l / T::LX) % T::LY, z = DIM == 3 ? l / T::LXY : 0;
-          const int ea = min(x / N, ti.ex - 1), eb = min(y / N, ti.ey - 1);
-          const int ez = DIM == 3 ? min(z / N, ti.ez - 1) : 0;
-          e = ea + TX * (eb + TY * ez);
-          p = (x - ea * N) + n * ((y - eb * N) + n * (z - ez * N));
-        }
-        const uint32_t g = valid ? (wk[r] & kNodeMask) : 0u;
-        const bool first = (wk[r] & kFlagFirst) != 0;
-        const bool fin = valid && (wk[r] & kFlagLast) && !A.ku_only;
-        mi[r] = fin ? gmi[g] : 0.f;
+      for (int r = 0; r < KB; ++r) load_slot(r0 + r, in[r]);
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-          const uint32_t idx = (s * C + c) * GG + g;
-          tot[r][c] = valid ? sK[(e * C + c) * ND + p] : 0.f;
-          prev[r][c] = (valid && !first) ? gacc[idx] : 0.f;
-          app[r][c] = fin ? gap[idx] : 0.f;
-          vv[r][c] = fin ? gv[idx] : 0.f;
-          kvv[r][c] = fin ? gkv[idx] : 0.f;
-          kuu[r][c] = fin ? gku[idx] : 0.f;
-        }
-      }
+      for (int r = 0; r < KB; ++r) finish_slot(r0 + r, in[r]);
+    }
+    if (!more) break;
+    // own slots only: u of this shot has been read above
 #pragma unroll
-      for (int r = 0; r < KB; ++r) {
-        if (wk[r] == kSlotEmpty) continue;
-        const int l = tid + (r0 + r) * T::BT;
-        const uint32_t g = wk[r] & kNodeMask;
-        const bool first = (wk[r] & kFlagFirst) != 0;
-        const bool fin = (wk[r] & kFlagLast) && !A.ku_only;
+    for (int r = 0; r < T::NIT; ++r) {
+      const int l = tid + r * T::BT;
+      if (l < T::NTL) {
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-          const uint32_t idx = (s * C + c) * GG + g;
-          const float t = first ? tot[r][c] : prev[r][c] + tot[r][c];
-          if (!fin) {
-            gacc[idx] = t;
-            continue;
-          }
-          if (q3 && c == 2) continue;
-          const float a = mi[r] * t;
-          float ap = app[r][c], v = vv[r][c], kv = kvv[r][c];
-          float u = su[c * T::NTL + l], ku = kuu[r][c];
-          const int reps = (q3 && c == 0) ? 2 : 1;
-          for (int rep = 0; rep < reps; ++rep) {
-            kahan(v, kv, G.acl * (a + ap));
-            kahan(u, ku, fmaf(G.dt, v, G.dsp * a));
-            ap = a;
-          }
-          gu[idx] = u; gv[idx] = v; gap[idx] = a; gku[idx] = ku; gkv[idx] = kv;
-        }
+        for (int c = 0; c < C; ++c) su[c * T::NTL + l] = nu[r][c];
       }
     }
+    __syncthreads();  // next lattice staged; sK and sF free again
   }
 }
 
