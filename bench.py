@@ -68,13 +68,14 @@ def tile_bytes_per_step(M, S, dim, physics):
     per shot      : u gather, 4c B per mesh slot (tile halos included);
                     accumulator 4c B per slot read (not its first colour) and
                     per slot written (not its last colour); per node 4 B
-                    inverse mass + c (16 B read v, a_, ku, kv + 20 B write u,
-                    v, a_, ku, kv)
+                    inverse mass + c (12 B read v, a_, kc + 16 B write u, v,
+                    a_, kc), kc being the packed pair of bf16 Kahan
+                    compensations of u and v
     """
     c = M.c
     per_tile = 4 * M.tile_slots + 4 * M.ntiles * M.tile_elems * elem_floats(dim, physics)
     per_shot = (4 * c * M.tile_valid_slots + 4 * c * (M.tile_acc_reads + M.tile_acc_writes)
-                + M.G * (4 + 36 * c))
+                + M.G * (4 + 28 * c))
     return per_tile + S * per_shot
 
 
@@ -86,11 +87,11 @@ def element_bytes_per_step(M, S, dim, physics):
                   read-modify-write) + the element's factors, here 4 B x
                   (affine factors + 2^d vertex material) per element instead of
                   per GLL point
-    per node    : S (4 B inverse mass + c (4 B a + 20 B read u, v, a_, ku, kv
-                  + 20 B write))
+    per node    : S (4 B inverse mass + c (4 B a + 16 B read u, v, a_, kc
+                  + 16 B write)), kc the packed Kahan compensations
     """
     c, nd = M.c, M.nd
-    return M.E * (nd * (4 + 12 * c * S) + 4 * elem_floats(dim, physics)) + M.G * S * (4 + 44 * c)
+    return M.E * (nd * (4 + 12 * c * S) + 4 * elem_floats(dim, physics)) + M.G * S * (4 + 36 * c)
 
 
 def setup(sem, workloads, name, nsteps, sizes, shots_per_mesh):
@@ -248,7 +249,7 @@ def main():
                      "kernel": "k_tile<%d,4,%s,%s>" % (dim, ("acoustic", "elastic")[physics], shape),
                      "algorithmic_bytes_per_launch": per_launch_bytes,
                      "bytes_model": "element formulation: per element n^d (4 + 12 c S) + 4 x "
-                                    "(affine factors + vertex material); per node S (4 + 44 c)",
+                                    "(affine factors + vertex material); per node S (4 + 36 c)",
                      "tile_bytes_per_launch": tile_launch_bytes,
                      "tile_achieved_GBs": tile_achieved,
                      "avg_launch_us": avg_launch_s * 1e6,

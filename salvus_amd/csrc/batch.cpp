@@ -74,9 +74,9 @@ int setup_group(sem_batch *b, int gi) {
       }
   }
   const size_t len = (size_t)m->c * m->G * g.S;
-  SEM_TRY(dalloc(&g.fields, 6 * len, &g.bytes));
+  SEM_TRY(dalloc(&g.fields, 5 * len, &g.bytes));
   std::vector<float> buf;
-  for (int which = 0; which < 6; ++which) {
+  for (int which = 0; which < 5; ++which) {  // u v ap kc acc; kc = 0 is two bf16 zeros
     bool any = false;
     if (which < 3)
       for (int sl = 0; sl < g.S; ++sl) any |= !b->shots[g.shots[sl]].init[which].empty();
@@ -188,9 +188,8 @@ int setup_group(sem_batch *b, int gi) {
   h.u = g.fields;
   h.v = g.fields + len;
   h.ap = g.fields + 2 * len;
-  h.ku = g.fields + 3 * len;
-  h.kv = g.fields + 4 * len;
-  h.acc = g.fields + 5 * len;
+  h.kc = reinterpret_cast<uint32_t *>(g.fields + 3 * len);
+  h.acc = g.fields + 4 * len;
   h.src = g.d_src;
   h.stf = g.d_stf;
   h.load = g.d_load;
@@ -476,7 +475,7 @@ int enqueue_plain_step(sem_batch *b) {
       SEM_TRY(launch_plain_scatter(m->col_start[c], m->col_start[c + 1] - m->col_start[c], m->nd, m->c,
                                    m->G, m->d_conn, b->d_aloc, h.acc, b->stream));
     SEM_TRY(launch_plain_update(m->G, m->c, m->quirks, h.dt, h.acl, h.dsp, m->d_mi, h.acc, h.u, h.v,
-                                h.ap, h.ku, h.kv, b->stream));
+                                h.ap, h.kc, b->stream));
   }
   SEM_TRY(enqueue_receivers(b, 0));
   SEM_TRY(launch_advance((int)b->groups.size(), b->d_groups, 1, b->stream));

@@ -54,7 +54,7 @@ struct Group {
   int S = 1;
   int64_t nsteps = 0;
   double dt = 0;
-  float *fields = nullptr;  // 6 arrays of c*G*S: u v ap ku kv acc
+  float *fields = nullptr;  // 5 arrays of c*G*S: u v ap kc acc
   SrcDev *d_src = nullptr;
   float *d_stf = nullptr, *d_load = nullptr;
   int nsrc = 0;

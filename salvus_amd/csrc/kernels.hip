@@ -288,8 +288,8 @@ struct NodeMat {
 };
 
 template <int DIM, int N, int PHYS, bool AFFINE>
-__device__ __forceinline__ void load_mat(const NodeCtx<DIM, N> &X, int64_t e, const float *geo_e,
-                                         const float *par_e, const float *geo_p,
+__device__ __forceinline__ void load_mat(const NodeCtx<DIM, N> &X, int64_t e, gptr<const float> geo_e,
+                                         gptr<const float> par_e, gptr<const float> geo_p,
                                          NodeMat<DIM, PHYS> &M) {
   using S = Sh<DIM, N>;
   if (AFFINE) {
@@ -298,7 +298,7 @@ __device__ __forceinline__ void load_mat(const NodeCtx<DIM, N> &X, int64_t e, co
     for (int v = 0; v < S::NV; ++v) w8[v] = p1_weight(DIM, v, X.xi);
     if (PHYS == SEM_ACOUSTIC) {
       constexpr int ng = DIM * (DIM + 1) / 2;
-      const float *gp = geo_e + e * ng;
+      const gptr<const float> gp = geo_e + e * ng;
       float vp = 0.f;
 #pragma unroll
       for (int v = 0; v < S::NV; ++v) vp = fmaf(w8[v], par_e[e * S::NV + v], vp);
@@ -307,12 +307,12 @@ __device__ __forceinline__ void load_mat(const NodeCtx<DIM, N> &X, int64_t e, co
       for (int k = 0; k < ng; ++k) M.G[k] = s * gp[k];
     } else {
       constexpr int ng = DIM * DIM + 1;
-      const float *gp = geo_e + e * ng;
+      const gptr<const float> gp = geo_e + e * ng;
 #pragma unroll
       for (int k = 0; k < DIM * DIM; ++k) M.Jinv[k] = gp[k];
       M.wdet = X.wq * gp[DIM * DIM];
       if (DIM == 2) {
-        const float *pp = par_e + e * 4 * 4;  // 4 vertices x {C11,C13,C33,C55}
+        const gptr<const float> pp = par_e + e * 4 * 4;  // 4 vertices x {C11,C13,C33,C55}
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           float s = 0.f;
@@ -321,7 +321,7 @@ __device__ __forceinline__ void load_mat(const NodeCtx<DIM, N> &X, int64_t e, co
           M.c[q] = s;
         }
       } else {
-        const float *pp = par_e + e * 8 * 6;  // 8 vertices x {RHO,VPV,VPH,VSV,VSH,ETA}
+        const gptr<const float> pp = par_e + e * 8 * 6;  // 8 vertices x {RHO,VPV,VPH,VSV,VSH,ETA}
         float q6[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
@@ -340,7 +340,7 @@ __device__ __forceinline__ void load_mat(const NodeCtx<DIM, N> &X, int64_t e, co
     }
   } else {
     constexpr int ngp = PHYS == SEM_ACOUSTIC ? DIM * (DIM + 1) / 2 : (DIM == 2 ? 9 : 19);
-    const float *gp = geo_p + e * ngp * S::ND + X.p;
+    const gptr<const float> gp = geo_p + e * ngp * S::ND + X.p;
     if (PHYS == SEM_ACOUSTIC) {
 #pragma unroll
       for (int k = 0; k < ngp; ++k) M.G[k] = gp[k * S::ND];
@@ -443,10 +443,12 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
   int item = bid;
   if (nb >= 16 && (nb & 7) == 0) item = (bid & 7) * (nb >> 3) + (bid >> 3);
   if (item >= A.nitems) return;
-  const WorkItem W = A.items[item];
-  const GroupDev &G = A.groups[W.group];
-  const int64_t step = *G.step + A.koff;
+  const WorkItem W = glb(A.items)[item];
+  const auto &G = glb(A.groups)[W.group];
+  const int64_t step = *glb(G.step) + A.koff;
   if (step >= G.nsteps) return;  // uniform across the block
+  const gptr<float> gu = glb(G.u), gv = glb(G.v), gap = glb(G.ap), gacc = glb(G.acc);
+  const gptr<uint32_t> gkc = glb(G.kc);
 
   const int t = threadIdx.x;
   const int el = t / S::ND;
@@ -462,16 +464,16 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
   float mi = 0.f;
   if (active) {
     X.init(p);
-    word = G.conn[e * S::ND + p];
+    word = glb(G.conn)[e * S::ND + p];
     g = word & kNodeMask;
-    load_mat<DIM, N, PHYS, AFFINE>(X, e, G.geo_e, G.par_e, G.geo_p, M);
-    if (word & kFlagLast) mi = G.mi[g];
+    load_mat<DIM, N, PHYS, AFFINE>(X, e, glb(G.geo_e), glb(G.par_e), glb(G.geo_p), M);
+    if (word & kFlagLast) mi = glb(G.mi)[g];
   }
   const int Sg = G.S;
   for (int s = 0; s < Sg; ++s) {
     if (active) {
 #pragma unroll
-      for (int c = 0; c < C; ++c) su[c * S::ND + p] = G.u[((int64_t)s * C + c) * G.G + g];
+      for (int c = 0; c < C; ++c) su[c * S::ND + p] = gu[((int64_t)s * C + c) * G.G + g];
     }
     __syncthreads();
     if (active) node_flux<DIM, N, PHYS>(X, M, su, sF);
@@ -484,21 +486,21 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
         f[c] = (word & kFlagDirichlet) ? 0.f : -ku;
       }
       for (int k = 0; k < G.nsrc; ++k) {
-        const SrcDev sd = G.src[k];
+        const SrcDev sd = glb(G.src)[k];
         if (sd.elem == e && sd.shot == s) {
-          float R = G.stf[sd.stf_off + step];
+          float R = glb(G.stf)[sd.stf_off + step];
 #pragma unroll
-          for (int c = 0; c < C; ++c) f[c] = fmaf(G.load[sd.load_off + c * S::ND + p], R, f[c]);
+          for (int c = 0; c < C; ++c) f[c] = fmaf(glb(G.load)[sd.load_off + c * S::ND + p], R, f[c]);
         }
       }
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         const int64_t idx = ((int64_t)s * C + c) * G.G + g;
         if (!(word & kFlagLast) || A.ku_only) {
-          if (word & kFlagFirst) G.acc[idx] = f[c];
-          else G.acc[idx] += f[c];
+          if (word & kFlagFirst) gacc[idx] = f[c];
+          else gacc[idx] += f[c];
         } else {
-          f[c] = (word & kFlagFirst) ? f[c] : G.acc[idx] + f[c];
+          f[c] = (word & kFlagFirst) ? f[c] : gacc[idx] + f[c];
         }
       }
       if ((word & kFlagLast) && !A.ku_only) {
@@ -509,16 +511,17 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
           if (q3 && c == 2) continue;
           const int64_t idx = ((int64_t)s * C + c) * G.G + g;
           float a = mi * f[c];
-          float ap = G.ap[idx];
-          float v = G.v[idx], kv = G.kv[idx];
-          float u = su[c * S::ND + p], ku = G.ku[idx];
+          float ap = gap[idx];
+          const uint32_t kc = gkc[idx];
+          float v = gv[idx], kv = comp_v(kc);
+          float u = su[c * S::ND + p], ku = comp_u(kc);
           const int reps = (q3 && c == 0) ? 2 : 1;
           for (int r = 0; r < reps; ++r) {
             kahan_add(v, kv, G.acl * (a + ap));
             kahan_add(u, ku, fmaf(G.dt, v, G.dsp * a));
             ap = a;
           }
-          G.u[idx] = u; G.v[idx] = v; G.ap[idx] = a; G.ku[idx] = ku; G.kv[idx] = kv;
+          gu[idx] = u; gv[idx] = v; gap[idx] = a; gkc[idx] = pack_comp(ku, kv);
         }
       }
     }
@@ -563,16 +566,17 @@ __global__ void __launch_bounds__(512) k_plain_stiffness(PlainArgs A) {
   NodeMat<DIM, PHYS> M;
   if (active) {
     X.init(p);
-    load_mat<DIM, N, PHYS, false>(X, e, nullptr, nullptr, A.geo_p, M);
+    load_mat<DIM, N, PHYS, false>(X, e, glb((const float *)nullptr), glb((const float *)nullptr),
+                                  glb(A.geo_p), M);
 #pragma unroll
-    for (int c = 0; c < C; ++c) su[c * S::ND + p] = A.uloc[(e * C + c) * S::ND + p];
+    for (int c = 0; c < C; ++c) su[c * S::ND + p] = glb(A.uloc)[(e * C + c) * S::ND + p];
   }
   __syncthreads();
   if (active) node_flux<DIM, N, PHYS>(X, M, su, sF);
   __syncthreads();
   if (!active) return;
-  const uint32_t word = A.conn[e * S::ND + p];
-  const int64_t step = A.step ? *A.step + A.koff : 0;
+  const uint32_t word = glb(A.conn)[e * S::ND + p];
+  const int64_t step = A.step ? *glb(A.step) + A.koff : 0;
   float f[C];
 #pragma unroll
   for (int c = 0; c < C; ++c) {
@@ -580,14 +584,15 @@ __global__ void __launch_bounds__(512) k_plain_stiffness(PlainArgs A) {
     f[c] = (word & kFlagDirichlet) ? 0.f : -ku;
   }
   for (int k = 0; k < A.nsrc; ++k) {
-    if (A.src[k].elem == e) {
-      float R = A.stf[A.src[k].stf_off + step];
+    const SrcDev sd = glb(A.src)[k];
+    if (sd.elem == e) {
+      float R = glb(A.stf)[sd.stf_off + step];
 #pragma unroll
-      for (int c = 0; c < C; ++c) f[c] = fmaf(A.load[A.src[k].load_off + c * S::ND + p], R, f[c]);
+      for (int c = 0; c < C; ++c) f[c] = fmaf(glb(A.load)[sd.load_off + c * S::ND + p], R, f[c]);
     }
   }
 #pragma unroll
-  for (int c = 0; c < C; ++c) A.aloc[(e * C + c) * S::ND + p] = f[c];
+  for (int c = 0; c < C; ++c) glb(A.aloc)[(e * C + c) * S::ND + p] = f[c];
 }
 
 __global__ void k_plain_scatter(int64_t first, int64_t count, int nd, int C, int64_t G,
@@ -604,7 +609,7 @@ __global__ void k_plain_scatter(int64_t first, int64_t count, int nd, int C, int
 __global__ void k_plain_update(int64_t G, int C, int quirks, float dt, float acl, float dsp,
                                const float *__restrict__ mi, const float *__restrict__ acc,
                                float *__restrict__ u, float *__restrict__ v, float *__restrict__ ap,
-                               float *__restrict__ ku, float *__restrict__ kv) {
+                               uint32_t *__restrict__ kc) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= G) return;
   const bool q3 = C == 3 && (quirks & SEM_QUIRK_NEWMARK3D);
@@ -613,14 +618,15 @@ __global__ void k_plain_update(int64_t G, int C, int quirks, float dt, float acl
     if (q3 && c == 2) continue;
     int64_t i = (int64_t)c * G + g;
     float a = m * acc[i];
-    float pa = ap[i], vv = v[i], kvv = kv[i], uu = u[i], kuu = ku[i];
+    const uint32_t k2 = kc[i];
+    float pa = ap[i], vv = v[i], kvv = comp_v(k2), uu = u[i], kuu = comp_u(k2);
     int reps = (q3 && c == 0) ? 2 : 1;
     for (int r = 0; r < reps; ++r) {
       kahan_add(vv, kvv, acl * (a + pa));
       kahan_add(uu, kuu, fmaf(dt, vv, dsp * a));
       pa = a;
     }
-    u[i] = uu; v[i] = vv; ap[i] = a; ku[i] = kuu; kv[i] = kvv;
+    u[i] = uu; v[i] = vv; ap[i] = a; kc[i] = pack_comp(kuu, kvv);
   }
 }
 
@@ -632,23 +638,23 @@ __global__ void k_receivers(RecArgs A, int C) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= A.nrec * C) return;
   int r = t / C, c = t % C;
-  const GroupDev &G = A.groups[A.rec_group[r]];
-  const int64_t step = *G.step + A.koff;
+  const auto &G = glb(A.groups)[glb(A.rec_group)[r]];
+  const int64_t step = *glb(G.step) + A.koff;
   if (step >= G.nsteps) return;
-  const RecDev R = A.recs[r];
-  const uint32_t *nodes = A.nodes + R.nodes_off;
+  const RecDev R = glb(A.recs)[r];
+  const gptr<const uint32_t> nodes = glb(A.nodes) + R.nodes_off;
+  const gptr<const float> u = glb(G.u) + ((int64_t)R.shot * C + c) * G.G;
   float s = 0.f;
-  for (int p = 0; p < A.nd; ++p)
-    s = fmaf(A.weights[R.w_off + p], G.u[((int64_t)R.shot * C + c) * G.G + nodes[p]], s);
-  R.out[step * R.stride + c] = s;
+  for (int p = 0; p < A.nd; ++p) s = fmaf(glb(A.weights)[R.w_off + p], u[nodes[p]], s);
+  glb(R.out)[step * R.stride + c] = s;
 }
 
 __global__ void k_advance(int ngroups, const GroupDev *groups, int64_t k) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ngroups) return;
-  int64_t *step = groups[i].step;
+  const gptr<int64_t> step = glb(glb(groups)[i].step);
   const int64_t s = *step + k;
-  *step = s < groups[i].nsteps ? s : groups[i].nsteps;
+  *step = s < glb(groups)[i].nsteps ? s : glb(groups)[i].nsteps;
 }
 
 }  // namespace sem
@@ -815,8 +821,8 @@ int launch_plain_scatter(int64_t first, int64_t count, int nd, int C, int64_t G,
 
 int launch_plain_update(int64_t G, int C, int quirks, float dt, float acl, float dsp,
                         const float *mi, const float *acc, float *u, float *v, float *ap,
-                        float *ku, float *kv, hipStream_t st) {
-  k_plain_update<<<nblocks(G, 256), 256, 0, st>>>(G, C, quirks, dt, acl, dsp, mi, acc, u, v, ap, ku, kv);
+                        uint32_t *kc, hipStream_t st) {
+  k_plain_update<<<nblocks(G, 256), 256, 0, st>>>(G, C, quirks, dt, acl, dsp, mi, acc, u, v, ap, kc);
   return last_launch("plain update");
 }
 

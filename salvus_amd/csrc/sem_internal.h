@@ -66,6 +66,38 @@ int colour_elements(int dim, int64_t E, int64_t V, const int32_t *cells,
                     std::vector<int32_t> &colour);
 
 // ---------------------------------------------------------------- device-side descriptors
+// Global-memory view of a device pointer.  Pointers that kernels read out of
+// structs in memory are generic, and generic (flat) accesses also count on
+// the LDS wait counter, so every wait for an LDS read would also wait for all
+// outstanding global loads; kernels access HBM through these views instead.
+// (The host pass parses kernel bodies too; there the view is the plain pointer.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+#else
+template <class T>
+using gptr = T *;
+#endif
+template <class T>
+__device__ __forceinline__ gptr<T> glb(T *p) {
+  return (gptr<T>)p;
+}
+
+// Kahan compensations of u and v are stored as the two bf16 halves of one
+// word (round to nearest even).  A compensation is the rounding error of the
+// last update, at most half an ulp of the sum, so its leading 8 bits keep the
+// compensated u and v within about one FP32 ulp over thousands of steps while
+// the state moves 8 bytes less per component and node.
+__device__ __forceinline__ uint32_t bf16_bits(float x) {
+  const uint32_t b = __float_as_uint(x);
+  return (b + 0x7fffu + ((b >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ uint32_t pack_comp(float cu, float cv) {
+  return bf16_bits(cu) | (bf16_bits(cv) << 16);
+}
+__device__ __forceinline__ float comp_u(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float comp_v(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+
 // Per-element factor counts.
 // Affine geometry (one Jacobian per element): acoustic Gsym (dim*(dim+1)/2);
 // elastic Jinv (dim^2) + detJ (1).
@@ -121,7 +153,9 @@ struct GroupDev {
   const float *par_e;     // E x nv x kpar
   const float *geo_p;     // pointwise: E x ngp x nd
   const float *mi;        // G
-  float *u, *v, *ap, *ku, *kv, *acc;
+  float *u, *v, *ap;
+  uint32_t *kc;           // Kahan compensations of u and v (bf16 pair per word)
+  float *acc;
   const SrcDev *src;
   const float *stf;
   const float *load;
@@ -236,7 +270,7 @@ int launch_plain_scatter(int64_t first, int64_t count, int nd, int C, int64_t G,
                          const uint32_t *conn, const float *aloc, float *acc, hipStream_t st);
 int launch_plain_update(int64_t G, int C, int quirks, float dt, float acl, float dsp,
                         const float *mi, const float *acc, float *u, float *v, float *ap,
-                        float *ku, float *kv, hipStream_t st);
+                        uint32_t *kc, hipStream_t st);
 int launch_receivers(const RecArgs &a, int C, hipStream_t st);
 // Advances every group's step counter by k (clamped at its nsteps).
 int launch_advance(int ngroups, const GroupDev *groups, int64_t k, hipStream_t st);

@@ -53,7 +53,7 @@ static double clock_of(const sem_server *s) {
 // Device bytes a request holds while active.
 static int64_t request_bytes(const sem_server *s, const Shot &sh) {
   const sem_mesh *m = sh.mesh;
-  int64_t b = 6 * 4 * (int64_t)m->c * m->G + 8;
+  int64_t b = 5 * 4 * (int64_t)m->c * m->G + 8;
   b += 4 * (int64_t)sh.recs.size() * m->c * sh.nsteps;
   b += (int64_t)sh.srcs.size() * (4 * sh.nsteps + 4 * (int64_t)m->nd * m->c + (int64_t)sizeof(SrcDev));
   if (s->path == SEM_PATH_PLAIN) b += 2 * 4 * m->E * (int64_t)m->nd * m->c;  // gather/scatter scratch

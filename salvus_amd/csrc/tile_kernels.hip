@@ -177,11 +177,11 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
   int item = bid;
   if (nb >= 16 && (nb & 7) == 0) item = (bid & 7) * (nb >> 3) + (bid >> 3);
   if (item >= A.nitems) return;
-  const WorkItem W = A.items[item];
-  const GroupDev &G = A.groups[W.group];
-  const int64_t step = *G.step + A.koff;
+  const WorkItem W = glb(A.items)[item];
+  const auto &G = glb(A.groups)[W.group];
+  const int64_t step = *glb(G.step) + A.koff;
   if (step >= G.nsteps) return;  // uniform across the block
-  const TileInfo ti = G.tiles[W.first];
+  const TileInfo ti = glb(G.tiles)[W.first];
   const int tid = threadIdx.x;
   const TileTables &TT = t_tab[N];
 
@@ -190,7 +190,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
   // the previous shot's face passes).  Outside the column phase each thread
   // writes and reads only its own slots l = tid + r BT of su.
   {
-    const uint32_t *__restrict__ words = G.tile_words + ti.node_off;
+    const gptr<const uint32_t> words = glb(G.tile_words) + ti.node_off;
     uint32_t wv[T::NIT];
 #pragma unroll
     for (int r = 0; r < T::NIT; ++r) {
@@ -203,11 +203,11 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
       const bool ok = wv[r] != kSlotEmpty;
       const uint32_t g = ok ? (wv[r] & kNodeMask) : 0u;
 #pragma unroll
-      for (int c = 0; c < C; ++c) uv[r][c] = ok ? G.u[c * (uint32_t)G.G + g] : 0.f;
+      for (int c = 0; c < C; ++c) uv[r][c] = ok ? glb(G.u)[c * (uint32_t)G.G + g] : 0.f;
     }
     for (int i = tid; i < n * n; i += T::BT) sD[i] = TT.D[i];
-    const float *__restrict__ tg = G.tgeo_e + (int64_t)ti.elem_off * T::NG;
-    const float *__restrict__ tp = G.tpar_e + (int64_t)ti.elem_off * (T::NV * T::KP);
+    const gptr<const float> tg = glb(G.tgeo_e) + (int64_t)ti.elem_off * T::NG;
+    const gptr<const float> tp = glb(G.tpar_e) + (int64_t)ti.elem_off * (T::NV * T::KP);
     for (int i = tid; i < T::E * T::NED; i += T::BT) {
       const int e = i / T::NED, k = i - e * T::NED;
       sE[i] = k < T::NG ? tg[e * T::NG + k] : tp[e * (T::NV * T::KP) + (k - T::NG)];
@@ -219,7 +219,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
         SrcDev sd{0, 0, 0, 0};
         bool in = false;
         if (j < G.nsrc) {
-          sd = G.src[j];
+          sd = glb(G.src)[j];
           in = sd.elem >= ti.elem_off && sd.elem < ti.elem_off + T::E;
         }
         const unsigned long long mask = __ballot(in);
@@ -257,13 +257,12 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
   const float xa = TT.xi[ci], xb = DIM == 3 ? TT.xi[cj] : 0.f;
   const int pl = ci + (DIM == 3 ? n * cj : 0);  // element-local index of the column's node 0
 
-  float *__restrict__ gu = G.u;
-  float *__restrict__ gv = G.v;
-  float *__restrict__ gap = G.ap;
-  float *__restrict__ gku = G.ku;
-  float *__restrict__ gkv = G.kv;
-  float *__restrict__ gacc = G.acc;
-  const float *__restrict__ gmi = G.mi;
+  const gptr<float> gu = glb(G.u);
+  const gptr<float> gv = glb(G.v);
+  const gptr<float> gap = glb(G.ap);
+  const gptr<uint32_t> gkc = glb(G.kc);
+  const gptr<float> gacc = glb(G.acc);
+  const gptr<const float> gmi = glb(G.mi);
   const uint32_t GG = (uint32_t)G.G, Sg = (uint32_t)G.S;
   const bool q3 = (C == 3) && (G.quirks & SEM_QUIRK_NEWMARK3D);
 
@@ -450,8 +449,8 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
         }
         for (int j = 0; j < nts; ++j) {
           if (sS[4 * j + 0] != e || sS[4 * j + 1] != (int)s) continue;
-          const float R = G.stf[sS[4 * j + 2] + step];
-          const float *ld = G.load + sS[4 * j + 3];
+          const float R = glb(G.stf)[sS[4 * j + 2] + step];
+          const gptr<const float> ld = glb(G.load) + sS[4 * j + 3];
 #pragma unroll
           for (int c = 0; c < C; ++c)
 #pragma unroll
@@ -472,7 +471,8 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
     // are loaded here so that their latency overlaps the face passes
     struct SlotIn {
       uint32_t w;
-      float mi, prev[C], ap[C], v[C], kv[C], ku[C];
+      float mi, prev[C], ap[C], v[C];
+      uint32_t kc[C];
     };
     auto load_slot = [&](int r, SlotIn &in) {
       const int l = tid + r * T::BT;
@@ -488,8 +488,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
         in.prev[c] = (valid && !first) ? gacc[idx] : 0.f;
         in.ap[c] = fin ? gap[idx] : 0.f;
         in.v[c] = fin ? gv[idx] : 0.f;
-        in.kv[c] = fin ? gkv[idx] : 0.f;
-        in.ku[c] = fin ? gku[idx] : 0.f;
+        in.kc[c] = fin ? gkc[idx] : 0u;
       }
     };
     constexpr int KF = T::NIT < 64 / (5 * C + 2) ? T::NIT : 64 / (5 * C + 2);
@@ -561,15 +560,15 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
         }
         if (q3 && c == 2) continue;
         const float a = in.mi * t;
-        float ap = in.ap[c], v = in.v[c], kv = in.kv[c];
-        float u = su[c * T::NTL + l], ku = in.ku[c];
+        float ap = in.ap[c], v = in.v[c], kv = comp_v(in.kc[c]);
+        float u = su[c * T::NTL + l], ku = comp_u(in.kc[c]);
         const int reps = (q3 && c == 0) ? 2 : 1;
         for (int rep = 0; rep < reps; ++rep) {
           kahan(v, kv, G.acl * (a + ap));
           kahan(u, ku, fmaf(G.dt, v, G.dsp * a));
           ap = a;
         }
-        gu[idx] = u; gv[idx] = v; gap[idx] = a; gku[idx] = ku; gkv[idx] = kv;
+        gu[idx] = u; gv[idx] = v; gap[idx] = a; gkc[idx] = pack_comp(ku, kv);
       }
     };
 #pragma unroll

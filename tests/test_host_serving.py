@@ -52,9 +52,9 @@ def test_byte_models_on_a_hand_case():
     # one 2-D acoustic element of order 1 (4 nodes), 1 tile of 1x1 elements
     M = types.SimpleNamespace(c=1, nd=4, E=1, G=4, ntiles=1, tile_elems=1, tile_slots=4,
                               tile_valid_slots=4, tile_acc_reads=0, tile_acc_writes=0)
-    # element formulation: 4 nodes x (4 + 12) + 4 x (3 factors + 4 vertex vp) + 4 x (4 + 44)
-    assert bench.element_bytes_per_step(M, 1, 2, 0) == 4 * 16 + 4 * 7 + 4 * 48
-    # tile algorithm: 4 words + 4 x 7 element floats + (4 u + 4 x (4 + 36))
-    assert bench.tile_bytes_per_step(M, 1, 2, 0) == 16 + 28 + 16 + 4 * 40
+    # element formulation: 4 nodes x (4 + 12) + 4 x (3 factors + 4 vertex vp) + 4 x (4 + 36)
+    assert bench.element_bytes_per_step(M, 1, 2, 0) == 4 * 16 + 4 * 7 + 4 * 40
+    # tile algorithm: 4 words + 4 x 7 element floats + (4 u + 4 x (4 + 28))
+    assert bench.tile_bytes_per_step(M, 1, 2, 0) == 16 + 28 + 16 + 4 * 32
     # S shots scale the per-shot parts only
-    assert bench.tile_bytes_per_step(M, 3, 2, 0) == 16 + 28 + 3 * (16 + 4 * 40)
+    assert bench.tile_bytes_per_step(M, 3, 2, 0) == 16 + 28 + 3 * (16 + 4 * 32)
