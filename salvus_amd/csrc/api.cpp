@@ -161,8 +161,7 @@ static int upload_tile_words(sem_mesh *m) {
 void sem::free_tiles(sem_mesh *m) {
   dfree(m->d_tile_info);
   dfree(m->d_tile_words);
-  dfree(m->d_tgeo_e);
-  dfree(m->d_tpar_e);
+  dfree(m->d_telem);
   dfree(m->d_mi_int);
   m->bytes -= std::min(m->bytes, m->tile_bytes);
   m->tile_bytes = 0;
@@ -216,13 +215,27 @@ int sem::upload_tiles(sem_mesh *m) {
       else for (int q = 0; q < 6; ++q) dst[q] = (float)src[q];
     }
   }
+  // affine factors (FP64 on the device, FP32 out), then one array of
+  // [affine factors | vertex material] per element for the tile kernel
+  const int ng = affine_geo_count(dim, m->physics), ned = ng + nv * kp;
   double *d_tv = nullptr;
+  float *d_tg = nullptr;
+  std::vector<float> tg(E * ng), te(E * ned);
   SEM_TRY(dalloc(&d_tv, tv.size()));
   int rc = hip_check(hipMemcpyAsync(d_tv, tv.data(), tv.size() * 8, hipMemcpyHostToDevice, m->stream), "upload");
-  if (!rc) rc = dalloc(&m->d_tgeo_e, (size_t)E * affine_geo_count(dim, m->physics), &m->tile_bytes);
-  if (!rc) rc = launch_affine_geometry(dim, m->physics, E, d_tv, m->d_tgeo_e, m->stream);
-  if (!rc) rc = dalloc(&m->d_tpar_e, tp.size(), &m->tile_bytes);
-  if (!rc) rc = hip_check(hipMemcpyAsync(m->d_tpar_e, tp.data(), tp.size() * 4, hipMemcpyHostToDevice, m->stream), "upload");
+  if (!rc) rc = dalloc(&d_tg, tg.size());
+  if (!rc) rc = launch_affine_geometry(dim, m->physics, E, d_tv, d_tg, m->stream);
+  if (!rc) rc = hip_check(hipMemcpyAsync(tg.data(), d_tg, tg.size() * 4, hipMemcpyDeviceToHost, m->stream), "download");
+  if (!rc) rc = hip_check(hipStreamSynchronize(m->stream), "sync");
+  if (!rc) {
+    for (int64_t i = 0; i < E; ++i) {
+      std::copy_n(&tg[i * ng], ng, &te[i * ned]);
+      std::copy_n(&tp[i * nv * kp], nv * kp, &te[i * ned + ng]);
+    }
+    rc = dalloc(&m->d_telem, te.size(), &m->tile_bytes);
+  }
+  if (!rc) rc = hip_check(hipMemcpyAsync(m->d_telem, te.data(), te.size() * 4, hipMemcpyHostToDevice, m->stream), "upload");
+  dfree(d_tg);
   // inverse mass in the internal numbering
   int32_t *d_to = nullptr;
   if (!rc) rc = dalloc(&d_to, (size_t)m->G);
@@ -248,8 +261,7 @@ static void mesh_free(sem_mesh *m) {
   dfree(m->d_vpar);
   dfree(m->d_tile_info);
   dfree(m->d_tile_words);
-  dfree(m->d_tgeo_e);
-  dfree(m->d_tpar_e);
+  dfree(m->d_telem);
   dfree(m->d_mi_int);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;

@@ -178,8 +178,7 @@ int setup_group(sem_batch *b, int gi) {
   if (g.tiled) {
     h.tiles = m->d_tile_info;
     h.tile_words = m->d_tile_words;
-    h.tgeo_e = m->d_tgeo_e;
-    h.tpar_e = m->d_tpar_e;
+    h.telem = m->d_telem;
   }
   h.geo_e = m->d_geo_e;
   h.par_e = m->d_par_e;

@@ -170,8 +170,7 @@ struct GroupDev {
   // tile kernel (structured meshes): fields in tile-major node numbering
   const TileInfo *tiles;
   const uint32_t *tile_words;
-  const float *tgeo_e;    // affine factors in tile element order
-  const float *tpar_e;    // vertex material in tile element order
+  const float *telem;     // per element in tile order: affine factors, then vertex material
 };
 
 // Work item of a colour launch: EPB elements of one group (element kernel),
@@ -320,7 +319,7 @@ struct sem_mesh {
   sem::TileSet tiles;
   sem::TileInfo *d_tile_info = nullptr;
   uint32_t *d_tile_words = nullptr;
-  float *d_tgeo_e = nullptr, *d_tpar_e = nullptr, *d_mi_int = nullptr;
+  float *d_telem = nullptr, *d_mi_int = nullptr;
   size_t bytes = 0;       // device bytes held (tile path included)
   size_t tile_bytes = 0;  // of which the tile path
 };

@@ -206,12 +206,8 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
       for (int c = 0; c < C; ++c) uv[r][c] = ok ? glb(G.u)[c * (uint32_t)G.G + g] : 0.f;
     }
     for (int i = tid; i < n * n; i += T::BT) sD[i] = TT.D[i];
-    const gptr<const float> tg = glb(G.tgeo_e) + (int64_t)ti.elem_off * T::NG;
-    const gptr<const float> tp = glb(G.tpar_e) + (int64_t)ti.elem_off * (T::NV * T::KP);
-    for (int i = tid; i < T::E * T::NED; i += T::BT) {
-      const int e = i / T::NED, k = i - e * T::NED;
-      sE[i] = k < T::NG ? tg[e * T::NG + k] : tp[e * (T::NV * T::KP) + (k - T::NG)];
-    }
+    const gptr<const float> te = glb(G.telem) + (int64_t)ti.elem_off * T::NED;
+    for (int i = tid; i < T::E * T::NED; i += T::BT) sE[i] = te[i];
     if (tid < 64) {  // ordered compaction by wave 0 (ballot + prefix count)
       int cnt = 0;
       for (int j0 = 0; j0 < G.nsrc; j0 += 64) {
