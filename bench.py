@@ -187,13 +187,13 @@ def main():
     t0 = time.perf_counter()
     dim, physics, meshes, shots, desc = setup(sem, workloads, args.workload, total, sizes,
                                               args.shots)
-    if not args.no_tune:  # tile shape (elements per workgroup) per mesh
-        for M in meshes:
-            M.autotune(reps=3)
+    per_mesh = [sum(1 for s in shots if s[0] == i) for i in range(len(meshes))]
+    if not args.no_tune:  # tile shape (elements per workgroup) per mesh, timed on
+        for M, k in zip(meshes, per_mesh):  # as many shots as a group will hold
+            M.autotune(reps=5, shots=max(1, min(k, args.spg or 4)))
     log("meshes ready (%.1f s): %s" % (time.perf_counter() - t0,
                                         ", ".join("E=%d tiles %s" % (M.E, M.tile_shape)
                                                   for M in meshes)))
-    per_mesh = [sum(1 for s in shots if s[0] == i) for i in range(len(meshes))]
     dofs = sum(M.c * M.G * k for M, k in zip(meshes, per_mesh))
 
     # ---------------- batched path

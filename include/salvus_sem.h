@@ -80,12 +80,14 @@ int sem_mesh_destroy(sem_mesh *mesh);
  *                slots, slots on the mesh, accumulator reads and writes per
  *                shot-step, elements per tile (padded), 0} */
 int sem_mesh_info(const sem_mesh *mesh, int64_t *info);
-/* Tile-shape autotuner (elements per workgroup of the tile kernel): times
- * `reps` full batched steps of one shot for every compiled tile shape of the
- * mesh's order and physics and keeps the fastest; *shape = Tx*10000 + Ty*100
- * + Tz (0: the mesh takes the element kernel).  No reference counterpart.
- * Call before the mesh joins a batch. */
-int sem_mesh_autotune(sem_mesh *mesh, int reps, int64_t *shape);
+/* Tile-shape autotuner (elements per workgroup of the tile kernel): for
+ * every compiled tile shape of the mesh's order and physics, times `reps`
+ * full batched steps of one group of `shots` shots (the batch size the mesh
+ * will serve; the best shape depends on it) twice and keeps the shape with
+ * the fastest minimum; *shape = Tx*10000 + Ty*100 + Tz (0: the mesh takes
+ * the element kernel).  No reference counterpart.  Call before the mesh
+ * joins a batch. */
+int sem_mesh_autotune(sem_mesh *mesh, int reps, int shots, int64_t *shape);
 /* Gather map conn[e*nd + p] = global DOF, element order as given
  * (DMPlexVecGetClosure + ClsMap, Problem.cpp:302-320). */
 int sem_mesh_get_conn(const sem_mesh *mesh, int32_t *conn);

@@ -867,10 +867,12 @@ int sem_batch_kernel_time(const sem_batch *b, double *ms, int64_t *launches) {
   return 0;
 }
 
-int sem_mesh_autotune(sem_mesh *m, int reps, int64_t *shape) {
+int sem_mesh_autotune(sem_mesh *m, int reps, int shots, int64_t *shape) {
   *shape = 0;
   HIPCHK(hipSetDevice(m->device));
   if (!m->tiles.valid) return 0;  // element kernel: nothing to choose per mesh
+  if (shots < 1 || shots > 64) { set_error("autotune shots must be 1..64"); return -1; }
+  reps = std::max(reps, 1);
   int shapes[3 * 16];
   const int ns = tile_kernel_shapes(m->dim, m->order, m->physics, shapes, 16);
   double best = 1e300;
@@ -880,10 +882,17 @@ int sem_mesh_autotune(sem_mesh *m, int reps, int64_t *shape) {
     if (!m->tiles.valid) continue;
     sem_batch *b = nullptr;
     SEM_TRY(sem_batch_create(&b, m->device));
-    int k = 0;
-    int rc = sem_batch_add_shot(b, m, reps + 2, 1.0, &k);
-    if (!rc) rc = sem_batch_prepare(b, SEM_PATH_FUSED, default_epb(m->nd), 1);
-    const double t = rc ? -1 : time_steps(b, std::max(reps, 1));
+    int rc = 0;
+    for (int s = 0; s < shots && !rc; ++s) {
+      int k = 0;
+      rc = sem_batch_add_shot(b, m, 2 * reps + 4, 1.0, &k);
+    }
+    if (!rc) rc = sem_batch_prepare(b, SEM_PATH_FUSED, default_epb(m->nd), shots);
+    double t = rc ? -1 : time_steps(b, reps);
+    if (t >= 0) {
+      const double t2 = time_steps(b, reps);
+      t = t2 < 0 ? t2 : std::min(t, t2);
+    }
     sem_batch_destroy(b);
     if (rc) return rc;
     if (t < 0) { set_error("autotune timing failed"); return -2; }

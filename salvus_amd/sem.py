@@ -102,7 +102,7 @@ def lib():
         L.sem_server_pump.argtypes = [_VP, _D]
         L.sem_server_request.argtypes = [_VP, C.c_int, _D]
         L.sem_server_result.argtypes = [_VP, C.c_int, _F]
-        L.sem_mesh_autotune.argtypes = [_VP, C.c_int, _I64]
+        L.sem_mesh_autotune.argtypes = [_VP, C.c_int, C.c_int, _I64]
         _lib = L
     return _lib
 
@@ -187,11 +187,12 @@ class Mesh:
         self.tile_acc_reads, self.tile_acc_writes = int(info[12]), int(info[13])
         self.tile_elems = int(info[14])
 
-    def autotune(self, reps=3):
-        """Times every compiled tile shape over `reps` steps and keeps the
+    def autotune(self, reps=5, shots=1):
+        """Times every compiled tile shape over `reps` steps of one group of
+        `shots` shots (the batch size the mesh will serve) and keeps the
         fastest (call before the mesh joins a batch); returns the shape."""
         v = np.zeros(1, dtype=np.int64)
-        _check(lib().sem_mesh_autotune(self._h, reps, _l(v)))
+        _check(lib().sem_mesh_autotune(self._h, reps, shots, _l(v)))
         self._read_info()
         return self.tile_shape
 

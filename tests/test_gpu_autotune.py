@@ -14,11 +14,11 @@ TOL_RUN = 1e-5
 COMPILED_3D = {(4, 2, 2), (2, 2, 2), (4, 4, 2)}
 
 
-@pytest.mark.parametrize("physics", [0, 1])
-def test_mesh_autotune_picks_compiled_shape(physics):
+@pytest.mark.parametrize("physics,shots", [(0, 1), (1, 1), (1, 3)])
+def test_mesh_autotune_picks_compiled_shape(physics, shots):
     P = pb.make(3, physics, 5, order=4, seed=40 + physics, nsteps=100)
     M = sem.Mesh(3, 4, physics, P["cells"], P["coords"], P["par"])
-    shape = M.autotune(reps=2)
+    shape = M.autotune(reps=2, shots=shots)
     assert shape in COMPILED_3D and M.ntiles > 0
     O = po.Sim(3, 4, physics, P["cells"], P["coords"], P["par"])
     s = P["src"]
