@@ -169,7 +169,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--no-tune", action="store_true", help="skip the tile-shape autotuner")
+    ap.add_argument("--tune", action="store_true",
+                    help="pick each mesh's tile shape with the autotuner (default: the "
+                         "library's default shape, so that every tile launch runs one kernel)")
     ap.add_argument("--spg", type=int, default=0, help="shots per group (0: autotuned)")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=400.0)
@@ -188,7 +190,7 @@ def main():
     dim, physics, meshes, shots, desc = setup(sem, workloads, args.workload, total, sizes,
                                               args.shots)
     per_mesh = [sum(1 for s in shots if s[0] == i) for i in range(len(meshes))]
-    if not args.no_tune:  # tile shape (elements per workgroup) per mesh, timed on
+    if args.tune:  # tile shape (elements per workgroup) per mesh, timed on
         for M, k in zip(meshes, per_mesh):  # as many shots as a group will hold
             M.autotune(reps=5, shots=max(1, min(k, args.spg or 4)))
     log("meshes ready (%.1f s): %s" % (time.perf_counter() - t0,
