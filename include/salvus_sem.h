@@ -123,6 +123,14 @@ int sem_batch_add_shot(sem_batch *batch, sem_mesh *mesh, int64_t nsteps, double 
  * whose closed hull holds it (Problem.cpp:57-113); dir has c entries. */
 int sem_shot_add_source(sem_batch *batch, int shot, const double *loc, double amp, double f0,
                         double t0, const double *dir);
+/* Tabulated point source (SourceHdf5.cpp:12-36 with the physics'
+ * computeSourceTerm, Scalar.cpp:79-88, Elastic2D.cpp:115-127,
+ * Elastic3D.cpp:126-133): at step n component k of the force is
+ * stf[k * nsamples + n], i.e. row n of the column-major nsamples x c matrix
+ * SourceHdf5::fire reads (time_idx = n); ncomp must equal the mesh's c and
+ * nsamples must cover the shot's steps. */
+int sem_shot_add_file_source(sem_batch *batch, int shot, const double *loc, int ncomp,
+                             int64_t nsamples, const double *stf);
 /* Moment-tensor Ricker source (no reference counterpart):
  * f_p = -M grad(phi_p)(loc) R(t); M is dim x dim row-major. */
 int sem_shot_add_moment_source(sem_batch *batch, int shot, const double *loc, const double *M,
@@ -142,6 +150,24 @@ int sem_batch_run(sem_batch *batch, int64_t nsteps);
 int sem_shot_progress(sem_batch *batch, int shot, int64_t *steps, double *time);
 /* Receiver samples nsteps x R x c (FP32), sampled after each step. */
 int sem_shot_get_seismograms(sem_batch *batch, int shot, float *out);
+
+/* ------------------------------------------------------------------ files
+ * HDF5 (libhdf5 >= 1.10 opened at run time: SEM_HDF5_LIB, else libhdf5.so,
+ * libhdf5.so.103 or /opt/conda/lib; host only, no GPU). */
+/* ReceiverHdf5::write (ReceiverHdf5.cpp:31-87): per receiver r a group
+ * /names[r] holding one FP32 dataset /names[r]/fields[k] of nsamples values
+ * for each of ncomp fields; data is nsamples x nrec x ncomp (the layout of
+ * sem_shot_get_seismograms).  append = 0 creates (truncates) the file,
+ * 1 adds to it (a receiver name already present is an error).  Names hold
+ * no '/'. */
+int sem_receivers_write_h5(const char *path, int append, int nrec, const char *const *names,
+                           int ncomp, const char *const *fields, int64_t nsamples,
+                           const float *data);
+/* SourceHdf5::loadData (SourceHdf5.cpp:40-50): reads dataset "<name>/data"
+ * as nsteps x ncomp doubles in the order stored (the reference's column-major
+ * matrix: component k's series is out[k * nsteps + n]); the dataset must hold
+ * exactly nsteps * ncomp values. */
+int sem_source_read_h5(const char *path, const char *name, int64_t nsteps, int ncomp, double *out);
 /* Launch configuration chosen: cfg[0..3] = {elems/block, shots/group,
  * graph block steps, groups}. */
 int sem_batch_config(const sem_batch *batch, int64_t *cfg);

@@ -586,7 +586,7 @@ int sem::source_load(const sem_mesh *m, const SrcSpec &s, int64_t *elem, std::ve
     lagrange_deriv(g, ref[a], dl[a]);
   }
   load.assign((size_t)nd * c, 0.0);
-  if (s.kind == 0) {
+  if (s.kind == 0 || s.kind == 2) {
     const bool e3d_quirk = dim == 3 && m->physics == SEM_ELASTIC && (m->quirks & SEM_QUIRK_E3D_SOURCE);
     for (int p = 0; p < nd; ++p) {
       int P[3] = {p % n, (p / n) % n, dim == 3 ? p / (n * n) : 0};

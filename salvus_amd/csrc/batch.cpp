@@ -67,7 +67,9 @@ int setup_group(sem_batch *b, int gi) {
         int64_t e;
         std::vector<double> ld;
         SEM_TRY(source_load(m, sp, &e, ld));
-        if (++per_tile[m->tiles.elem_pos[e] / m->tiles.elems_per_tile] > kTileMaxSrc) {
+        int &cnt = per_tile[m->tiles.elem_pos[e] / m->tiles.elems_per_tile];
+        cnt += sp.kind == 2 ? m->c : 1;  // a file source is one source per component
+        if (cnt > kTileMaxSrc) {
           g.tiled = false;
           break;
         }
@@ -113,22 +115,36 @@ int setup_group(sem_batch *b, int gi) {
     Shot &s = b->shots[g.shots[sl]];
     s.group = gi;
     s.slot = sl;
-    for (auto &sp : s.srcs) {
-      int64_t e;
-      std::vector<double> ld;
-      SEM_TRY(source_load(m, sp, &e, ld));
-      SrcDev d;
-      d.elem = g.tiled ? m->tiles.elem_pos[e] : m->inv_perm[e];
-      d.stf_off = (int32_t)stf.size();
-      d.load_off = (int32_t)load.size();
-      d.shot = sl;
-      double t = 0.0;
-      for (int64_t k = 0; k < g.nsteps; ++k) {
-        stf.push_back((float)ricker(t, sp.amp, sp.f0, sp.t0));
-        t += g.dt;
+    for (auto &sp0 : s.srcs) {
+      // a file source is one unit-direction source per component with that
+      // component's tabulated series as its source-time function
+      const int parts = sp0.kind == 2 ? m->c : 1;
+      for (int k = 0; k < parts; ++k) {
+        SrcSpec sp = sp0;
+        if (sp0.kind == 2) {
+          sp.kind = 0;
+          for (int q = 0; q < 3; ++q) sp.dir[q] = q == k ? 1.0 : 0.0;
+        }
+        int64_t e;
+        std::vector<double> ld;
+        SEM_TRY(source_load(m, sp, &e, ld));
+        SrcDev d;
+        d.elem = g.tiled ? m->tiles.elem_pos[e] : m->inv_perm[e];
+        d.stf_off = (int32_t)stf.size();
+        d.load_off = (int32_t)load.size();
+        d.shot = sl;
+        if (sp0.kind == 2) {
+          for (int64_t n = 0; n < g.nsteps; ++n) stf.push_back((float)sp0.series[k * sp0.nser + n]);
+        } else {
+          double t = 0.0;
+          for (int64_t n = 0; n < g.nsteps; ++n) {
+            stf.push_back((float)ricker(t, sp.amp, sp.f0, sp.t0));
+            t += g.dt;
+          }
+        }
+        for (double v : ld) load.push_back((float)v);
+        src.push_back(d);
       }
-      for (double v : ld) load.push_back((float)v);
-      src.push_back(d);
     }
     // receivers: Lagrange weights of the owning element at the receiver
     s.seis_off = seis;
@@ -635,6 +651,26 @@ int sem_shot_add_source(sem_batch *b, int shot, const double *loc, double amp, d
   std::vector<double> load;
   SEM_TRY(source_load(s->mesh, sp, &e, load));  // validates placement now
   s->srcs.push_back(sp);
+  return 0;
+}
+
+int sem_shot_add_file_source(sem_batch *b, int shot, const double *loc, int ncomp,
+                             int64_t nsamples, const double *stf) {
+  Shot *s = get_shot(b, shot);
+  if (!s) return -1;
+  if (b->prepared) { set_error("batch already prepared"); return -1; }
+  if (ncomp != s->mesh->c) { set_error("file source needs one series per field component"); return -1; }
+  if (nsamples < s->nsteps || !stf) { set_error("file source needs a sample for every step"); return -1; }
+  SrcSpec sp{};
+  sp.kind = 2;
+  for (int a = 0; a < s->mesh->dim; ++a) sp.loc[a] = loc[a];
+  sp.dir[0] = 1.0;
+  int64_t e;
+  std::vector<double> load;
+  SEM_TRY(source_load(s->mesh, sp, &e, load));  // validates placement now
+  sp.nser = nsamples;
+  sp.series.assign(stf, stf + (size_t)ncomp * nsamples);
+  s->srcs.push_back(std::move(sp));
   return 0;
 }
 

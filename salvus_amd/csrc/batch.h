@@ -31,8 +31,12 @@ inline void dfree(T *&p) {
 }
 
 struct SrcSpec {
-  int kind;  // 0 force, 1 moment tensor
+  int kind;  // 0 force (Ricker), 1 moment tensor (Ricker), 2 file (tabulated)
   double loc[3], amp, f0, t0, dir[3], M[9];
+  // kind 2: component k's force at step n is series[k * nser + n]
+  // (SourceHdf5.cpp:31-36: the rows of a column-major nsteps x c matrix)
+  int64_t nser = 0;
+  std::vector<double> series;
 };
 
 struct Shot {

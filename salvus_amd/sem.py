@@ -38,7 +38,8 @@ EXPORTS = (
     "sem_shot_get_seismograms", "sem_batch_config", "sem_batch_kernel_time",
     "sem_reference_step_count", "sem_server_create", "sem_server_destroy", "sem_server_clock",
     "sem_server_submit", "sem_server_pump", "sem_server_request", "sem_server_result",
-    "sem_mesh_autotune",
+    "sem_mesh_autotune", "sem_shot_add_file_source", "sem_receivers_write_h5",
+    "sem_source_read_h5",
 )
 SOURCE_DOUBLES = 19
 
@@ -103,6 +104,11 @@ def lib():
         L.sem_server_request.argtypes = [_VP, C.c_int, _D]
         L.sem_server_result.argtypes = [_VP, C.c_int, _F]
         L.sem_mesh_autotune.argtypes = [_VP, C.c_int, C.c_int, _I64]
+        L.sem_shot_add_file_source.argtypes = [_VP, C.c_int, _D, C.c_int, C.c_int64, _D]
+        L.sem_receivers_write_h5.argtypes = [C.c_char_p, C.c_int, C.c_int,
+                                             C.POINTER(C.c_char_p), C.c_int,
+                                             C.POINTER(C.c_char_p), C.c_int64, _F]
+        L.sem_source_read_h5.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, C.c_int, _D]
         _lib = L
     return _lib
 
@@ -140,6 +146,36 @@ def reference_step_count(duration, dt):
     out = np.zeros(1)
     n = lib().sem_reference_step_count(duration, dt, _d(out))
     return int(n), float(out[0])
+
+
+FIELD_NAMES = {1: ("u",), 2: ("ux", "uy"), 3: ("ux", "uy", "uz")}
+
+
+def _cstrings(items):
+    arr = (C.c_char_p * len(items))()
+    arr[:] = [s.encode() for s in items]
+    return arr
+
+
+def write_receivers_h5(path, names, seis, fields=None, append=False):
+    """ReceiverHdf5::write format: /<name>/<field> FP32 datasets.  seis is
+    nsamples x nrec x c (Batch.seismograms); fields default to the physics'
+    field names (u; ux, uy; ux, uy, uz)."""
+    seis = np.ascontiguousarray(seis, dtype=np.float32)
+    if seis.ndim != 3 or seis.shape[1] != len(names):
+        raise ValueError("seis must be nsamples x len(names) x c")
+    fields = tuple(fields) if fields is not None else FIELD_NAMES[seis.shape[2]]
+    _check(lib().sem_receivers_write_h5(str(path).encode(), int(bool(append)), len(names),
+                                        _cstrings(list(names)), len(fields), _cstrings(list(fields)),
+                                        seis.shape[0], _f(seis)))
+
+
+def read_source_h5(path, name, nsteps, ncomp):
+    """SourceHdf5::loadData: dataset <name>/data as an nsteps x ncomp matrix
+    (stored column-major, as the reference reads it)."""
+    out = np.zeros(int(nsteps) * int(ncomp))
+    _check(lib().sem_source_read_h5(str(path).encode(), name.encode(), nsteps, ncomp, _d(out)))
+    return out.reshape(ncomp, nsteps).T.copy()
 
 
 def topology(dim, order, cells, nvert):
@@ -284,6 +320,16 @@ class Batch:
             self._h, shot, _d(np.asarray(loc, dtype=np.float64)),
             _d(np.ascontiguousarray(M, dtype=np.float64)), amp, f0, t0))
 
+    def add_file_source(self, shot, loc, stf):
+        """Tabulated point source: stf[n, k] is force component k at step n
+        (SourceHdf5::fire row n); stf has one column per field component."""
+        stf = np.asarray(stf, dtype=np.float64)
+        if stf.ndim == 1:
+            stf = stf[:, None]
+        cols = np.ascontiguousarray(stf.T)  # component-major = column-major rows
+        _check(lib().sem_shot_add_file_source(self._h, shot, _d(np.asarray(loc, dtype=np.float64)),
+                                              stf.shape[1], stf.shape[0], _d(cols)))
+
     def add_receiver(self, shot, loc):
         _check(lib().sem_shot_add_receiver(self._h, shot, _d(np.asarray(loc, dtype=np.float64))))
         self.shots[shot][2] += 1
@@ -315,6 +361,10 @@ class Batch:
         _check(lib().sem_shot_get_seismograms(self._h, shot, _f(out)))
         return out[: nsteps * nrec * m.c].reshape(nsteps, nrec, m.c)
 
+    def write_receivers_h5(self, shot, path, names, append=False):
+        """Seismograms of one shot in the reference's receiver file format."""
+        write_receivers_h5(path, names, self.seismograms(shot), append=append)
+
     def config(self):
         c = np.zeros(4, dtype=np.int64)
         _check(lib().sem_batch_config(self._h, _l(c)))
@@ -343,6 +393,9 @@ class Order2Newmark:
 
     def add_moment_source(self, loc, M, amp, f0, t0):
         self.batch.add_moment_source(self.shot, loc, M, amp, f0, t0)
+
+    def add_file_source(self, loc, stf):
+        self.batch.add_file_source(self.shot, loc, stf)
 
     def add_receiver(self, loc):
         self.batch.add_receiver(self.shot, loc)

@@ -1,0 +1,90 @@
+"""Tabulated (file) sources and receiver files on the GPU: a file source whose
+series is the Ricker wavelet times the direction reproduces the Ricker
+source's seismograms (the Ricker path is pinned to the FP64 oracle by the
+parity tests), through the tile and the element kernels; a source read from
+an HDF5 file runs and the seismograms written to an HDF5 receiver file read
+back unchanged."""
+import numpy as np
+import pytest
+
+from salvus_amd import sem
+from tests import problems as pb
+from tests.test_hdf5io import H5, read_dataset, write_source_file
+
+pytestmark = pytest.mark.gpu
+
+
+def ricker_series(src, dt, nsteps, c):
+    """The Ricker source's per-step samples (Ricker.cpp:27-33) at the times the
+    step loop accumulates, times each direction component: nsteps x c."""
+    out = np.zeros((nsteps, c))
+    t = 0.0
+    f2 = np.pi ** 2 * src["f0"] ** 2
+    for n in range(nsteps):
+        fac = f2 * (t - src["t0"]) ** 2
+        out[n] = src["amp"] * (1 - 2 * fac) * np.exp(-fac) * np.asarray(src["dir"])[:c]
+        t += dt
+    return out
+
+
+def run(M, P, file_stf=None):
+    prob = sem.Order2Newmark(M, P["dt"], P["nsteps"])
+    s = P["src"]
+    if file_stf is None:
+        prob.add_source(s["loc"], s["amp"], s["f0"], s["t0"], s["dir"])
+    else:
+        prob.add_file_source(s["loc"], file_stf)
+    for r in P["recs"]:
+        prob.add_receiver(r)
+    prob.takeTimeStep(P["nsteps"])
+    return prob.seismograms()
+
+
+@pytest.mark.parametrize("dim,physics,distorted", [(2, 1, False), (3, 0, False), (3, 1, False),
+                                                   (2, 0, True)])
+def test_file_source_matches_ricker(dim, physics, distorted):
+    P = pb.make(dim, physics, 6 if dim == 3 else 12, order=4, seed=70 + dim + physics,
+                distorted=distorted, nsteps=120)
+    M = sem.Mesh(dim, 4, physics, P["cells"], P["coords"], P["par"])
+    assert (M.ntiles > 0) == (not distorted)
+    ref = run(M, P)
+    got = run(M, P, ricker_series(P["src"], P["dt"], P["nsteps"], P["c"]))
+    assert np.abs(ref).max() > 0
+    assert pb.rel_l2(got, ref) < 1e-6
+
+
+def test_file_source_needs_every_step():
+    P = pb.make(2, 1, 8, order=4, seed=80, nsteps=50)
+    M = sem.Mesh(2, 4, 1, P["cells"], P["coords"], P["par"])
+    prob = sem.Order2Newmark(M, P["dt"], P["nsteps"])
+    with pytest.raises(sem.SemError, match="every step"):
+        prob.add_file_source(P["src"]["loc"], np.zeros((49, 2)))
+    with pytest.raises(sem.SemError, match="component"):
+        prob.add_file_source(P["src"]["loc"], np.zeros((50, 3)))
+
+
+@pytest.mark.skipif(H5 is None, reason="libhdf5 not found on this host")
+def test_hdf5_source_to_hdf5_receivers(tmp_path):
+    P = pb.make(3, 1, 5, order=4, seed=81, nsteps=90)
+    M = sem.Mesh(3, 4, 1, P["cells"], P["coords"], P["par"])
+    series = ricker_series(P["src"], P["dt"], P["nsteps"], 3)
+    src_file = tmp_path / "source.h5"
+    write_source_file(src_file, {"shot_a": series.T.ravel()})  # column-major, as the reference
+    stf = sem.read_source_h5(src_file, "shot_a", P["nsteps"], 3)
+    np.testing.assert_array_equal(stf, series)
+    B = sem.Batch()
+    k = B.add_shot(M, P["nsteps"], P["dt"])
+    B.add_file_source(k, P["src"]["loc"], stf)
+    names = ["rec%d" % i for i in range(len(P["recs"]))]
+    for r in P["recs"]:
+        B.add_receiver(k, r)
+    B.prepare(sem.PATH_FUSED)
+    B.run(P["nsteps"])
+    seis = B.seismograms(k)
+    rec_file = tmp_path / "receivers.h5"
+    B.write_receivers_h5(k, rec_file, names)
+    for i, name in enumerate(names):
+        for c, field in enumerate(("ux", "uy", "uz")):
+            np.testing.assert_array_equal(read_dataset(rec_file, "/%s/%s" % (name, field))[0],
+                                          seis[:, i, c])
+    assert np.abs(seis).max() > 0
