@@ -10,9 +10,10 @@
 // element's reference axes follow the lattice axes.
 //
 // Tiles are coloured by lattice parity (2^dim colours; tiles of one colour
-// share no node).  Nodes are renumbered tile-major: each tile's new nodes in
-// slot order, its inside first and then one run per upper face, edge and
-// corner, so that a tile's own nodes and each piece its neighbours share are
+// share no node).  Nodes are renumbered tile-major by the tile that finishes
+// them (the last colour holding the node): each tile's nodes in slot order,
+// its inside first and then one run per lower/upper face, edge and corner, so
+// that a tile's Newmark state and each piece its neighbours share are
 // contiguous.
 #include <algorithm>
 #include <cmath>
@@ -99,26 +100,7 @@ int build_tiles(sem_mesh *m, int Tx, int Ty, int Tz, TileSet &ts) {
               }
             }
       }
-  // ---- internal numbering: tile-major; a tile's new nodes come in runs by
-  // which of its upper faces they lie on (none, +x, +y, +x+y edge, +z, ...),
-  // so that every face, edge or corner a neighbouring tile reads or
-  // accumulates is one contiguous range
-  ts.to_internal.assign(m->G, -1);
-  int64_t next = 0;
-  for (int64_t t = 0; t < ntiles; ++t) {
-    const TileInfo &ti = ts.info[t];
-    const int fx = ti.ex * N, fy = ti.ey * N, fz = ti.ez * N;  // last lattice index
-    for (int cat = 0; cat < (1 << dim); ++cat)
-      for (int64_t s = 0; s < NTL; ++s) {
-        const int32_t g = slot_node[t * NTL + s];
-        if (g < 0 || ts.to_internal[g] >= 0) continue;
-        const int x = (int)(s % LX), y = (int)((s / LX) % LY), z = (int)(s / ((int64_t)LX * LY));
-        const int c = (x == fx) | ((y == fy) << 1) | ((dim == 3 && z == fz) << 2);
-        if (c == cat) ts.to_internal[g] = (int32_t)next++;
-      }
-  }
-  if (next != m->G) return 1;
-  // ---- per node: min/max tile colour -> first/last flags per slot
+  // ---- per node: min/max colour of the tiles holding it (first/last flags)
   std::vector<int8_t> cmin(m->G, 127), cmax(m->G, -1);
   for (int64_t t = 0; t < ntiles; ++t)
     for (int64_t s = t * NTL; s < (t + 1) * NTL; ++s) {
@@ -127,6 +109,40 @@ int build_tiles(sem_mesh *m, int Tx, int Ty, int Tz, TileSet &ts) {
       cmin[g] = std::min<int8_t>(cmin[g], (int8_t)ts.info[t].colour);
       cmax[g] = std::max<int8_t>(cmax[g], (int8_t)ts.info[t].colour);
     }
+  // ---- internal numbering, tile-major: each node belongs to the tile that
+  // finishes it (the last colour holding it), so a tile's Newmark reads and
+  // writes are one contiguous range; in it the nodes come in runs by position
+  // (inside, then each lower/upper face, edge and corner), so every piece a
+  // neighbouring tile gathers or accumulates is contiguous too.
+  ts.to_internal.assign(m->G, -1);
+  int64_t next = 0;
+  // Per tile the eligible slots are bucketed by category (a counting sort in
+  // slot order), then numbered bucket by bucket.
+  const int ncat = dim == 3 ? 27 : 9;
+  std::vector<int32_t> slot_cat(NTL), order(NTL);
+  std::vector<int64_t> start(ncat + 1);
+  for (int64_t t = 0; t < ntiles; ++t) {
+    const TileInfo &ti = ts.info[t];
+    const int fx = ti.ex * N, fy = ti.ey * N, fz = ti.ez * N;  // last lattice index
+    std::fill(start.begin(), start.end(), 0);
+    for (int64_t s = 0; s < NTL; ++s) {
+      const int32_t g = slot_node[t * NTL + s];
+      slot_cat[s] = -1;
+      if (g < 0 || ts.to_internal[g] >= 0 || cmax[g] != ti.colour) continue;
+      const int x = (int)(s % LX), y = (int)((s / LX) % LY), z = (int)(s / ((int64_t)LX * LY));
+      const int px = x == 0 ? 1 : x == fx ? 2 : 0, py = y == 0 ? 1 : y == fy ? 2 : 0;
+      const int pz = dim == 3 ? (z == 0 ? 1 : z == fz ? 2 : 0) : 0;
+      const int c = px + 3 * py + 9 * pz;
+      slot_cat[s] = c;
+      ++start[c + 1];
+    }
+    for (int c = 0; c < ncat; ++c) start[c + 1] += start[c];
+    const int64_t cnt = start[ncat];
+    for (int64_t s = 0; s < NTL; ++s)
+      if (slot_cat[s] >= 0) order[start[slot_cat[s]]++] = (int32_t)s;
+    for (int64_t i = 0; i < cnt; ++i) ts.to_internal[slot_node[t * NTL + order[i]]] = (int32_t)next++;
+  }
+  if (next != m->G) return 1;
   ts.words.assign(ntiles * NTL, kSlotEmpty);
   for (int64_t t = 0; t < ntiles; ++t)
     for (int64_t s = t * NTL; s < (t + 1) * NTL; ++s) {
