@@ -1,0 +1,66 @@
+"""Vertex material from elemental model variables, the way the reference's
+ExodusModel serves them to the elements.
+
+Each element asks for every parameter at each of its vertices
+(TensorQuad.cpp:243-251, Hexahedra.cpp:414-425); the model answers with the
+elemental variable "<NAME>_<vertex>" of the model element whose centre is
+nearest the element's centre (a k-d tree over model element centres,
+ExodusModel.cpp:422-471), where the centre is the vertex mean
+(TensorQuad.cpp:239, Hexahedra.cpp:157).  An isotropic "VP" request falls
+back to "VPV" and an anisotropic "VPV" request to "VP"
+(ExodusModel.cpp:440-456).  The physics ask for (Scalar.cpp:24,
+Elastic2D.cpp:29-36, Elastic3D.cpp:33-40):
+
+  acoustic         VP
+  Elastic2D        RHO, C11, C13, C33, C55
+  Elastic3D        RHO, VPV, VPH, VSV, VSH, ETA
+
+`vertex_parameters` returns the E x nv x npar array that `sem.Mesh` takes.
+Reading the Exodus file itself is outside this module: pass the model's
+element centres and its elemental variables as arrays.
+"""
+import numpy as np
+from scipy.spatial import cKDTree
+
+PARAMETERS = {
+    (2, 0): ("VP",),
+    (3, 0): ("VP",),
+    (2, 1): ("RHO", "C11", "C13", "C33", "C55"),
+    (3, 1): ("RHO", "VPV", "VPH", "VSV", "VSH", "ETA"),
+}
+
+
+def element_centres(cells, coords):
+    """Vertex mean of every element (cells: E x nv vertex ids)."""
+    return np.asarray(coords, dtype=np.float64)[np.asarray(cells)].mean(axis=1)
+
+
+def resolve_name(name, variables):
+    """The elemental variable family a parameter request is served from."""
+    if name + "_0" in variables:
+        return name
+    if name == "VP" and "VPV_0" in variables:
+        return "VPV"
+    if name == "VPV" and "VP_0" in variables:
+        return "VP"
+    raise KeyError("Requested parameter %s which is not stored as an elemental variable" % name)
+
+
+def vertex_parameters(dim, physics, cells, coords, model_centres, variables):
+    """E x nv x npar vertex material for sem.Mesh.
+
+    model_centres: M x dim centres of the model's elements; variables:
+    {"<NAME>_<v>": length-M array} for every parameter the physics needs and
+    every vertex v."""
+    if not variables:
+        raise KeyError("You've tried to query a elemental parameter, but none are defined.")
+    cells = np.asarray(cells)
+    names = [resolve_name(n, variables) for n in PARAMETERS[(dim, physics)]]
+    _, nearest = cKDTree(np.asarray(model_centres, dtype=np.float64)).query(
+        element_centres(cells, coords))
+    nv = cells.shape[1]
+    out = np.empty((len(cells), nv, len(names)))
+    for q, name in enumerate(names):
+        for v in range(nv):
+            out[:, v, q] = np.asarray(variables["%s_%d" % (name, v)], dtype=np.float64)[nearest]
+    return out
