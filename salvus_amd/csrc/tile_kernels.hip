@@ -114,9 +114,13 @@ struct TS {
   static constexpr int CS = DIM == 3 ? LXY : LX;            // lattice stride of the column axis
   static constexpr int ES = DIM == 3 ? n * n : n;           // element-local stride of the column axis
   static constexpr int E = TX * TY * (DIM == 3 ? TZ : 1);
-  static constexpr int EC = E < 512 / NCOL ? E : 512 / NCOL;  // elements per column chunk
-  static constexpr int NCH = (E + EC - 1) / EC;
-  static constexpr int BT = ((EC * NCOL + 63) / 64) * 64;   // threads
+  // column chunks of at most 512 threads, all chunks equally full; the
+  // workgroup keeps the threads of the largest chunk (they all gather and
+  // finish lattice slots)
+  static constexpr int EC0 = E < 512 / NCOL ? E : 512 / NCOL;
+  static constexpr int NCH = (E + EC0 - 1) / EC0;
+  static constexpr int EC = (E + NCH - 1) / NCH;             // elements per column chunk
+  static constexpr int BT = ((EC0 * NCOL + 63) / 64) * 64;  // threads
   static constexpr int NIT = (NTL + BT - 1) / BT;           // lattice slots per thread
   static constexpr int NG = PHYS == SEM_ACOUSTIC ? DIM * (DIM + 1) / 2 : DIM * DIM + 1;
   static constexpr int KP = PHYS == SEM_ACOUSTIC ? 1 : (DIM == 2 ? 4 : 6);
