@@ -6,8 +6,12 @@
 // One thread owns one node column of one element: the column runs along the
 // last reference axis (eta in 2-D, zeta in 3-D), so the derivative along it
 // is a register-only product with wave-uniform D entries, and only the
-// lateral derivatives read the LDS lattice.  Per shot:
-//   gather  u of every lattice slot -> LDS (loads batched per thread);
+// lateral derivatives read the LDS lattice.  Columns run in chunks of at most
+// 512 threads, all chunks equally full.  Per shot:
+//   gather  u of every lattice slot -> LDS (the first shot's with the tile's
+//           words and element data; every later shot's, with the finish
+//           inputs of the first slots, during the previous shot's face
+//           passes, so each shot waits for one memory round trip);
 //   column  reference gradient and J^-T flux per node; lateral fluxes go to
 //           LDS, the column flux stays in registers; the test-and-integrate
 //           result (-Ku, zero on Dirichlet rows, plus source loads) is
