@@ -633,20 +633,25 @@ __global__ void k_plain_update(int64_t G, int C, int quirks, float dt, float acl
 // ------------------------------------------------------------------ receivers / steps
 
 
-// Samples u after the step just taken (global step = *step + koff).
-__global__ void k_receivers(RecArgs A, int C) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= A.nrec * C) return;
-  int r = t / C, c = t % C;
+// Samples u after the step just taken (global step = *step + koff): one wave
+// per receiver component, its lanes over the owning element's nodes, summed
+// by a fixed butterfly.
+__global__ void __launch_bounds__(256) k_receivers(RecArgs A, int C) {
+  const int t = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t >= A.nrec * C) return;  // uniform per wave
+  const int r = t / C, c = t % C;
   const auto &G = glb(A.groups)[glb(A.rec_group)[r]];
   const int64_t step = *glb(G.step) + A.koff;
   if (step >= G.nsteps) return;
   const RecDev R = glb(A.recs)[r];
   const gptr<const uint32_t> nodes = glb(A.nodes) + R.nodes_off;
+  const gptr<const float> w = glb(A.weights) + R.w_off;
   const gptr<const float> u = glb(G.u) + ((int64_t)R.shot * C + c) * G.G;
   float s = 0.f;
-  for (int p = 0; p < A.nd; ++p) s = fmaf(glb(A.weights)[R.w_off + p], u[nodes[p]], s);
-  glb(R.out)[step * R.stride + c] = s;
+  for (int p = lane; p < A.nd; p += 64) s = fmaf(w[p], u[nodes[p]], s);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) glb(R.out)[step * R.stride + c] = s;
 }
 
 __global__ void k_advance(int ngroups, const GroupDev *groups, int64_t k) {
@@ -828,7 +833,7 @@ int launch_plain_update(int64_t G, int C, int quirks, float dt, float acl, float
 
 int launch_receivers(const RecArgs &a, int C, hipStream_t st) {
   if (a.nrec <= 0) return 0;
-  k_receivers<<<nblocks((int64_t)a.nrec * C, 128), 128, 0, st>>>(a, C);
+  k_receivers<<<nblocks((int64_t)a.nrec * C, 4), 256, 0, st>>>(a, C);  // 4 waves per block
   return last_launch("receivers");
 }
 
