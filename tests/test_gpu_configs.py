@@ -2,7 +2,8 @@
 case (cfg1: 2-D acoustic 50x50 quads, order 4, 1000 steps) at full size, and
 the other configurations' laws (material, source type, receiver layout) on
 sub-boxes the FP64 oracle runs in seconds (cfg2: 100^2 elastic quads; cfg3:
-16^3 acoustic hexes; cfg4: 10^3 elastic hexes with a moment-tensor source).
+16^3 acoustic hexes, and the 32^3 sub-box for 200 steps; cfg4: 10^3 elastic
+hexes with a moment-tensor source).
 Both FP32 HIP paths must reach receiver and final-wavefield relative L2 of
 1e-5 against the oracle."""
 import os
@@ -17,7 +18,7 @@ from tests import problems as pb
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
-THREADS = min(os.cpu_count() or 1, 8)
+THREADS = min(os.cpu_count() or 1, 16)
 
 
 def sample(name):
@@ -30,10 +31,14 @@ def sample(name):
         mw = W["meshes"][0]
         return dict(dim=3, physics=1, cells=mw["cells"], coords=mw["coords"], par=mw["par"],
                     dt=mw["dt"], nsteps=250, sources=mw["shots"], receivers=mw["receivers"])
-    W = workloads.cfg3(n=16, nsteps=300, nrec_side=4) if name == "cfg3" else \
-        workloads.cfg2(n=100, nsteps=300)
+    if name == "cfg3-32":
+        W = workloads.cfg3(n=32, nsteps=200, nrec_side=8)
+    elif name == "cfg3":
+        W = workloads.cfg3(n=16, nsteps=300, nrec_side=4)
+    else:
+        W = workloads.cfg2(n=100, nsteps=300)
     return dict(dim=W["dim"], physics=W["physics"], cells=W["cells"], coords=W["coords"],
-                par=W["par"], dt=W["dt"], nsteps=300, sources=W["sources"],
+                par=W["par"], dt=W["dt"], nsteps=W["nsteps"], sources=W["sources"],
                 receivers=W["receivers"])
 
 
@@ -48,7 +53,7 @@ def add_all(target, sm, shot=None):
         target.add_receiver(*args, r)
 
 
-@pytest.mark.parametrize("name", ["cfg1", "cfg2", "cfg3", "cfg4"])
+@pytest.mark.parametrize("name", ["cfg1", "cfg2", "cfg3", "cfg3-32", "cfg4"])
 def test_config_parity(name):
     sm = sample(name)
     O = po.Sim(sm["dim"], 4, sm["physics"], sm["cells"], sm["coords"], sm["par"])

@@ -18,7 +18,7 @@ from tests import refmeshes as rm
 pytestmark = pytest.mark.gpu
 
 TOL_OP = 2e-6    # one FP32 operator application vs FP64
-TOL_RUN = 1e-5   # receiver / wavefield relative L2 over a run (BASELINE metric bar)
+TOL_RUN = 1e-5   # receiver / wavefield relative L2 over a run (the metric's accuracy bar)
 
 
 def oracle_for(P, quirks=po.DEFAULT_QUIRKS):

@@ -1,4 +1,5 @@
-"""Pins the oracle's element operators and physics:
+"""Pins the oracle's element operators and physics (gradient, test-and-integrate,
+material at GLL points, the three physics laws, sources, mass):
 the reference's own element identities (test_TensorQuad.cpp:71-206,
 test_TensorHex.cpp:65-240, test_source.cpp), plus closed forms that a wrong
 index, sign or material mapping would break."""

@@ -1,4 +1,5 @@
-"""Pins the oracle's P1 geometry with the golden
+"""Pins the oracle's P1 geometry (Jacobians, material interpolation, point
+location) with the golden
 values of the reference's test_QuadP1.cpp and test_HexP1.cpp."""
 import numpy as np
 import pytest

@@ -1,4 +1,4 @@
-"""Pins the oracle's GLL tables against the reference's compiled
+"""Pins the oracle's GLL tables (points, weights, derivative matrix) against the reference's compiled
 sympy tables, the closed form and exact polynomial identities."""
 import ctypes as C
 

@@ -1,4 +1,4 @@
-"""Pins plex-lite: DMPlex point numbering against the
+"""Pins plex-lite (gather map, scatter order, global numbering): DMPlex point numbering against the
 reference's test_Mesh.cpp facts, the closure gather map against the 7-hex
 known-answer test of test_TensorHex.cpp:400-603, and the invariants that
 hold for any correct gather map."""

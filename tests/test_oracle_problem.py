@@ -1,4 +1,5 @@
-"""Pins the oracle's whole time loop
+"""Pins the oracle's whole time loop (assembly, Dirichlet rows, mass, Newmark,
+step count)
 against the reference's end-to-end eigenfunction tests
 (test_Scalar_2D.cpp:238-308, test_Scalar_3D.cpp:152-235), closed-form
 elastic standing waves, and the exact algebra of its Newmark scheme."""
