@@ -177,7 +177,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=400.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs one process on one MI355X; "
+        print("bench.py measures one GPU: the library runs one process on one MI355X; "
               "--gpus %d is not supported (use --gpus 1)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

@@ -36,7 +36,10 @@ typedef struct sem_server sem_server;
 
 enum { SEM_ACOUSTIC = 0, SEM_ELASTIC = 1 };
 
-/* Reference-behaviour switches, see DESIGN: Q7 and Q20. */
+/* Reference-behaviour switches: SEM_QUIRK_E3D_SOURCE keeps Elastic3D's source
+ * scaling (its source term skips test-and-integrate, Elastic3D.cpp:126-133);
+ * SEM_QUIRK_NEWMARK3D advances Elastic3D's x twice and z never, as the field
+ * list {ax, ay, ax, a} of Order2Newmark.cpp:93-96 does. */
 enum { SEM_QUIRK_NEWMARK3D = 1, SEM_QUIRK_E3D_SOURCE = 2 };
 #define SEM_DEFAULT_QUIRKS SEM_QUIRK_E3D_SOURCE
 

@@ -27,7 +27,8 @@ int orc_tables(orc_tab *T, int dim, int order) {
 
 /* ---------------- quads ---------------- */
 
-/* TensorQuad.cpp:176-210; writes detJ per point (Q9). */
+/* TensorQuad.cpp:176-210; writes detJ per point (TensorQuad.cpp:194: the
+ * gradient leaves mDetJac for applyGradTestAndIntegrate, :371-374). */
 static void quad_gradient(const orc_tab *T, const double *vtx, const double *u, double *grad,
                           double *detJ) {
   int n = T->n;
@@ -358,7 +359,7 @@ int orc_elem_grad_test_integrate(int dim, int order, const double *vtx, const do
   orc_tab T;
   if (orc_tables(&T, dim, order)) return -1;
   if (dim == 2) {
-    /* detJ as computeGradient leaves it (Q9) */
+    /* detJ as computeGradient leaves it (TensorQuad.cpp:194, :371-374) */
     double detJ[ORC_NDMAX], g[2 * ORC_NDMAX], z[ORC_NDMAX];
     memset(z, 0, sizeof z);
     quad_gradient(&T, vtx, z, g, detJ);

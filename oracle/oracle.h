@@ -110,7 +110,8 @@ void orc_sim_conn(const orc_sim *S, int32_t *conn);
 void orc_sim_inverse_mass(const orc_sim *S, double *mi);
 /* Global node coordinates (G x d) from element-local nodal points. */
 void orc_sim_node_coords(const orc_sim *S, double *xyz);
-/* Homogeneous Dirichlet: zero stiffness rows at masked global nodes (Q14). */
+/* Homogeneous Dirichlet: zero stiffness rows at masked global nodes; sources,
+ * u and v are untouched (HomogeneousDirichlet.cpp:71-78). */
 int orc_sim_set_dirichlet(orc_sim *S, const uint8_t *node_mask);
 /* Ricker point source with direction dir[c] (Ricker.cpp:27-33). */
 int orc_sim_add_source(orc_sim *S, const double *loc, double amp, double f0, double t0,

@@ -279,7 +279,7 @@ void orc_plex_offsets(const orc_plex *P, int64_t *off) {
   memcpy(off, P->offset, sizeof(int64_t) * P->npoints);
 }
 
-/* Greedy colouring (build-defined; the reference has no colouring). */
+/* Greedy colouring (defined here; the reference has no colouring). */
 int orc_color_elements(int dim, int64_t E, int64_t V, const int32_t *cells, int32_t *colour) {
   int nv = dim == 2 ? 4 : 8;
   int64_t *start = (int64_t *)calloc(V + 1, sizeof(int64_t));

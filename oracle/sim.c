@@ -9,7 +9,7 @@
  * :68-83 (a <- mi a), :85-115 (v += dt/2 (a + a_); u += dt v + dt^2/2 a;
  * a_ = a; t += dt), HomogeneousDirichlet.cpp:71-78 (zero stiffness rows),
  * Main.cpp:45-66 (step loop), Ricker.cpp:27-33.  Receiver sampling is
- * build-defined (the reference never records, Q11): s = sum_p l_p(xr) u_p
+ * defined here (the reference locates receivers but never records them): s = sum_p l_p(xr) u_p
  * in the owning element, taken after each step's update.
  */
 #include <math.h>
@@ -138,7 +138,7 @@ orc_sim *orc_sim_new(int dim, int order, int physics, int64_t E, int64_t V,
   for (int64_t e = 0; e < E; e++) {
     const double *vx = S->vtx + e * nv * dim;
     if (physics == ORC_ACOUSTIC) {
-      orc_test_integrate(&S->T, vx, one, me); /* Scalar.cpp:34-39 (no rho, Q3) */
+      orc_test_integrate(&S->T, vx, one, me); /* Scalar.cpp:34-39 (the acoustic mass has no rho) */
     } else {
       orc_par_at_pts(&S->T, S->vpar + e * nv * S->npar, S->npar, rho); /* RHO is parameter 0 */
       orc_test_integrate(&S->T, vx, rho, me);
@@ -239,7 +239,7 @@ int orc_sim_add_source(orc_sim *S, const double *loc, double amp, double f0, dou
   return S->nsrc - 1;
 }
 
-/* Moment-tensor source (build-defined, Komatitsch & Tromp 1999):
+/* Moment-tensor source (defined here, Komatitsch & Tromp 1999):
  * f_{p,i} = -sum_j M_ij dphi_p/dx_j (xs) R(t). */
 int orc_sim_add_moment_source(orc_sim *S, const double *loc, const double *M, double amp,
                               double f0, double t0) {

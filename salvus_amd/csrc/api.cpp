@@ -596,7 +596,8 @@ int sem::source_load(const sem_mesh *m, const SrcSpec &s, int64_t *elem, std::ve
         wq *= g.w[P[a]];
       }
       // getDeltaFunctionCoefficients divides by w detJ at the node; the
-      // test-and-integrate of Scalar/Elastic2D multiplies it back (Q7).
+      // test-and-integrate of Scalar/Elastic2D multiplies it back; Elastic3D's
+      // source term skips it (Elastic3D.cpp:126-133).
       double r3[3] = {g.xi[P[0]], g.xi[P[1]], dim == 3 ? g.xi[P[2]] : 0.0}, J[9];
       double det = jacobian(dim, vx, r3, J);
       double delta = lp / (wq * det);

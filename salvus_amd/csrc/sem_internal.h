@@ -19,7 +19,7 @@ constexpr int kMaxN = 11;
 
 // Gather-map words carry three flags above the node index (G < 2^29).
 constexpr uint32_t kNodeMask = (1u << 29) - 1;
-constexpr uint32_t kFlagDirichlet = 1u << 29;  // zero the stiffness row (Q14)
+constexpr uint32_t kFlagDirichlet = 1u << 29;  // zero the stiffness row only
 constexpr uint32_t kFlagFirst = 1u << 30;      // first colour to touch the node: store
 constexpr uint32_t kFlagLast = 1u << 31;       // last colour: finalise (mass + Newmark)
 // Tile lattice slot outside the mesh (partial tile at the mesh edge).

@@ -1,6 +1,6 @@
 """Synthetic inputs of the five benchmark configurations (seeded).
 
-Every choice the reference does not fix is marked "build choice".  The
+Every choice the reference does not fix is marked "chosen here".  The
 arrays follow the reference's input conventions (cells in QuadP1/HexP1
 vertex order; per-element, per-vertex material in the reference's
 parameter order).
@@ -24,7 +24,7 @@ GLL4_MIN_SPACING = (1.0 - np.sqrt(3.0 / 7.0)) / 2.0  # smallest GLL gap / h at o
 
 def gaussian_field(shape, sigma_cells, rng):
     """White noise smoothed by a Gaussian of sigma_cells grid cells, scaled
-    to max |value| = 1 (build choice: noise on the vertex grid)."""
+    to max |value| = 1 (chosen here: noise on the vertex grid)."""
     from scipy.ndimage import gaussian_filter
     f = gaussian_filter(rng.standard_normal(shape), sigma_cells, mode="reflect")
     return f / np.abs(f).max()
@@ -49,12 +49,12 @@ def cfg2(n=500, h=10.0, nsteps=5000, nrec=100, seed=2):
     cells, coords = meshgen.box_quad(n, n, L, L)
     d = 0.05 * gaussian_field((n + 1, n + 1), 50.0 / h, rng).ravel()  # +-5 %, 50 m
     vp, vs, rho = 3000.0 * (1 + d), 1732.0 * (1 + d), 2500.0 * (1 + d)
-    # Q6: C11 = C33 = rho vp^2, C13 = rho (vp^2 - 2 vs^2), C55 = rho vs^2
+    # Elastic2D.cpp:53-82: C11 = C33 = rho vp^2, C13 = rho (vp^2 - 2 vs^2), C55 = rho vs^2
     c11 = rho * vp ** 2
     par = np.stack([rho, c11, rho * (vp ** 2 - 2 * vs ** 2), c11, rho * vs ** 2], axis=1)[cells]
     f0 = 30.0
     src = dict(kind="force", loc=(0.5 * L + 0.37 * h, 0.5 * L + 0.41 * h), amp=1e9, f0=f0,
-               t0=1.2 / f0, dir=(0.0, 1.0))  # vertical force (Q13: direction set here)
+               t0=1.2 / f0, dir=(0.0, 1.0))  # vertical force (the reference's CLI cannot set a direction)
     recs = [(0.1 * L + 0.8 * L * k / (nrec - 1), 0.7 * L) for k in range(nrec)]
     return dict(name="cfg2: 2-D elastic %dx%d quads of %g m, order 4" % (n, n, h), dim=2,
                 physics=1, order=4, cells=cells, coords=coords, par=par, dt=2e-4,
@@ -69,7 +69,7 @@ def cfg3(n=128, h=15.0, nsteps=2000, nrec_side=32, seed=3):
     layer = np.minimum((z / L * 8).astype(int), 7)
     vp_v = (2000.0 + 3000.0 * layer / 7.0) * (1 + rng.uniform(-0.03, 0.03, len(coords)))
     par = vp_v[cells][..., None]
-    dt = 0.4 * h * GLL4_MIN_SPACING / vp_v.max()  # build-defined CFL (Q18)
+    dt = 0.4 * h * GLL4_MIN_SPACING / vp_v.max()  # CFL number chosen here (the reference has no CFL logic)
     f0 = 20.0
     c = 0.5 * L
     src = dict(kind="force", loc=(c + 0.37 * h, c + 0.41 * h, c + 0.29 * h), amp=1e6, f0=f0,
@@ -82,8 +82,8 @@ def cfg3(n=128, h=15.0, nsteps=2000, nrec_side=32, seed=3):
 
 
 def elastic3d_mesh(n, h=15.0, seed=4):
-    """cfg4 mesh: seeded smooth vp around 3000 m/s (100 m Gaussian, build
-    choice), vs = vp/1.73, rho = 310 vp^0.25 (Gardner, build choice)."""
+    """cfg4 mesh: seeded smooth vp around 3000 m/s (100 m Gaussian, chosen
+    here), vs = vp/1.73, rho = 310 vp^0.25 (Gardner, chosen here)."""
     rng = np.random.default_rng(seed + n)
     L = n * h
     cells, coords = meshgen.box_hex(n, n, n, L, L, L)
@@ -91,7 +91,7 @@ def elastic3d_mesh(n, h=15.0, seed=4):
     vp = 3000.0 * (1 + d)
     vs = vp / 1.73
     rho = 310.0 * vp ** 0.25
-    par = np.stack([rho, vp, vp, vs, vs, np.ones_like(vp)], axis=1)[cells]  # Q19 isotropic
+    par = np.stack([rho, vp, vp, vs, vs, np.ones_like(vp)], axis=1)[cells]  # isotropic: VPV = VPH, VSV = VSH, ETA = 1 (Elastic3D.cpp:42-52)
     dt = 0.4 * h * GLL4_MIN_SPACING / vp.max()
     return cells, coords, par, dt, L
 

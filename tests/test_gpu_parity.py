@@ -53,8 +53,9 @@ def test_mesh_maps_bit_exact(dim, physics):
 @pytest.mark.parametrize("distorted", [False, True])
 def test_stiffness_operator(dim, physics, order, distorted):
     if distorted and dim == 2:
-        pytest.skip("non-affine quads: reference applies J^-1 at output nodes (Q8) with a "
-                    "swapped-coordinate Jacobian (Q21); product uses the consistent form")
+        pytest.skip("non-affine quads: reference applies J^-1 at output nodes "
+                    "(TensorQuad.cpp:383-390) with a swapped-coordinate Jacobian (QuadP1.h:20-27); "
+                    "product uses the consistent form")
     P = pb.make(dim, physics, 4 if dim == 3 else 7, order=order, seed=order, distorted=distorted)
     M = mesh_for(P)
     S = oracle_for(P)

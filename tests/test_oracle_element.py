@@ -166,7 +166,8 @@ def test_source_term_sums_to_amplitude(dim, physics):
 
 
 def test_elastic3d_source_quirk():
-    """Q7: Elastic3D's load is l_i(xs)/(w_i detJ_i), Scalar's l_i(xs)."""
+    """Elastic3D's source term skips applyTestAndIntegrate (Elastic3D.cpp:126-133):
+    its load is l_i(xs)/(w_i detJ_i), Scalar's l_i(xs)."""
     cells, coords = meshgen.box_hex(1, 1, 1, 2.0, 2.0, 2.0)
     par = np.tile([1.0, 2.0, 2.0, 1.0, 1.0, 1.0], 8)
     loc = [0.7, 1.1, 0.9]

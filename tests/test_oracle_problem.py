@@ -103,7 +103,7 @@ def test_elastic_standing_p_wave(dim):
 
 
 def test_step_count_quirk():
-    """Q2: dt = duration/ceil(duration/dt) and while(t < duration) with t
+    """Options.cpp:84-97, Main.cpp:45: dt = duration/ceil(duration/dt) and while(t < duration) with t
     accumulated in FP64 (Options.cpp:84-97, Main.cpp:45)."""
     assert po.reference_step_count(0.2, 2e-4)[0] == 1000  # configuration 1
     n, dt = po.reference_step_count(1.0, 2e-4)  # configuration 2 duration
