@@ -174,8 +174,11 @@ int sem_source_read_h5(const char *path, const char *name, int64_t nsteps, int n
 /* Launch configuration chosen: cfg[0..3] = {elems/block, shots/group,
  * graph block steps, groups}. */
 int sem_batch_config(const sem_batch *batch, int64_t *cfg);
-/* Timing of the dominant kernel (fused/stiffness) measured with HIP events
- * on the batch stream during the last sem_batch_run: total ms and count. */
+/* Timing of the dominant kernel during the last sem_batch_run when the
+ * environment sets SEM_TIME_KERNELS=1 (steps then run without graphs): total
+ * ms and number of launches, from HIP events on the batch stream (batched
+ * path: one pair around each step's colour launches of the tile or element
+ * kernel; plain path: one pair around each stiffness launch). */
 int sem_batch_kernel_time(const sem_batch *batch, double *ms, int64_t *launches);
 
 /* Reference time-loop semantics (Options.cpp:84-97, Main.cpp:45): number

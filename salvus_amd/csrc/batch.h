@@ -114,7 +114,8 @@ struct sem_batch {
   bool timing = false;
   double kernel_ms = 0;
   int64_t kernel_launches = 0;
-  std::vector<hipEvent_t> ev;
+  std::vector<hipEvent_t> ev;  // timing: event pairs around timed kernels
+  int64_t ev_launches = 0;       // kernel launches the event pairs cover
 };
 
 namespace sem {
