@@ -146,6 +146,18 @@ def sample(workloads, name):
                 receivers=W["receivers"], desc=desc)
 
 
+def cpu_model():
+    """The host CPU's model name (first /proc/cpuinfo entry), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def profiled_traffic(name):
     """HBM bytes per tile-kernel launch from the committed rocprofv3 summary
     of this workload (FETCH_SIZE x fetch factor + WRITE_SIZE x write factor),
@@ -329,6 +341,7 @@ def main():
         out["cpu_baseline_all_cores"] = {"value": dof_s / cpu_all_s, "unit": "DOF-updates/s",
                                          "cores": nthr, "kind": "port",
                                          "sample": "the same, colour-parallel (%.1f s)" % cpu_all_s}
+        out["cpu_host"] = {"nproc": os.cpu_count(), "model": cpu_model()}
 
     # ---------------- serving sample: continuous batching vs plain FIFO
     if args.workload == "cfg4" and not args.no_serve:
