@@ -504,11 +504,16 @@ class Server:
         _check(lib().sem_server_result(self._h, rid, _f(out)))
         return out[: nsteps * nrec * mesh.c].reshape(nsteps, nrec, mesh.c)
 
-    def drain(self, idle_sleep=0.0005):
-        """Pumps until every submitted request is finished."""
+    def drain(self, idle_sleep=0.0005, progress=None, every=10.0):
+        """Pumps until every submitted request is finished; progress(status)
+        is called about every `every` seconds of the server clock."""
         import time
+        last = self.clock()
         while True:
             st = self.pump()
+            if progress is not None and self.clock() - last >= every:
+                last = self.clock()
+                progress(st)
             if st["active"] == 0 and st["pending"] == 0:
                 return
             if st["active"] == 0 and st["next_arrival"] > 0:

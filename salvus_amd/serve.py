@@ -103,9 +103,10 @@ def _sources(model, req):
     return [dict(kind="force", loc=loc, amp=1e6, f0=f0, t0=1.2 / f0, dir=direction)]
 
 
-def run(models, reqs, path=sem.PATH_FUSED, max_active=0, block_steps=32, hbm_budget=0):
+def run(models, reqs, path=sem.PATH_FUSED, max_active=0, block_steps=32, hbm_budget=0,
+        progress=None):
     """Serves the stream in real time; returns per-request latencies and the
-    aggregate throughput."""
+    aggregate throughput.  progress(status) is called every ~10 s."""
     S = sem.Server(path=path, hbm_budget=hbm_budget, max_active=max_active,
                    block_steps=block_steps)
     t_start = S.clock()
@@ -114,7 +115,7 @@ def run(models, reqs, path=sem.PATH_FUSED, max_active=0, block_steps=32, hbm_bud
         m = models[r["model"]]
         ids.append(S.submit(m["mesh"], r["nsteps"], m["dt"], _sources(m, r), m["recs"],
                             arrival=t_start + r["arrival"]))
-    S.drain()
+    S.drain(progress=progress)
     lat, dofs, t_last = [], 0, t_start
     for r, i in zip(reqs, ids):
         info = S.request(i)
@@ -147,9 +148,14 @@ def main():
     specs, reqs = draw_stream(args.requests, args.rate, args.seed, args.models, args.scale,
                               (lo, hi))
     models = [build_model(s["dim"], s["physics"], s["n"], s["seed"]) for s in specs]
-    out = {"batched": run(models, reqs)[0]}
+
+    def progress(tag):
+        return lambda st: print("[serve %s] done %d, active %d, pending %d" %
+                                (tag, st["done"], st["active"], st["pending"]), flush=True)
+    out = {"batched": run(models, reqs, progress=progress("batched"))[0]}
     if args.plain:
-        out["plain"] = run(models, reqs, path=sem.PATH_PLAIN, max_active=1)[0]
+        out["plain"] = run(models, reqs, path=sem.PATH_PLAIN, max_active=1,
+                           progress=progress("plain"))[0]
     out["models"] = [dict(dim=s["dim"], physics=s["physics"], n=s["n"]) for s in specs]
     print(json.dumps(out))
 
