@@ -124,7 +124,13 @@ struct TS {
   static constexpr int EC0 = E < 512 / NCOL ? E : 512 / NCOL;
   static constexpr int NCH = (E + EC0 - 1) / EC0;
   static constexpr int EC = (E + NCH - 1) / NCH;             // elements per column chunk
-  static constexpr int BT = ((EC0 * NCOL + 63) / 64) * 64;  // threads
+  static constexpr int BT0 = ((EC0 * NCOL + 63) / 64) * 64;
+  // Threads: at least those of a chunk, and enough that every thread's
+  // lattice slots fit the registers that hold finish inputs loaded ahead
+  // (KFMAX slots), so that no slot waits for its loads in the finish.
+  static constexpr int KFMAX = 64 / (5 * C + 2);
+  static constexpr int BTF = (((NTL + KFMAX - 1) / KFMAX + 63) / 64) * 64;
+  static constexpr int BT = BTF > BT0 ? (BTF < 1024 ? BTF : 1024) : BT0;
   static constexpr int NIT = (NTL + BT - 1) / BT;           // lattice slots per thread
   static constexpr int NG = PHYS == SEM_ACOUSTIC ? DIM * (DIM + 1) / 2 : DIM * DIM + 1;
   static constexpr int KP = PHYS == SEM_ACOUSTIC ? 1 : (DIM == 2 ? 4 : 6);
@@ -495,7 +501,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
         in.kc[c] = fin ? gkc[idx] : 0u;
       }
     };
-    constexpr int KF = T::NIT < 64 / (5 * C + 2) ? T::NIT : 64 / (5 * C + 2);
+    constexpr int KF = T::NIT < T::KFMAX ? T::NIT : T::KFMAX;
     SlotIn pre[KF];
 #pragma unroll
     for (int r = 0; r < KF; ++r) load_slot(r, pre[r]);
