@@ -470,10 +470,22 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
     if (word & kFlagLast) mi = glb(G.mi)[g];
   }
   const int Sg = G.S;
+  const bool first = (word & kFlagFirst) != 0;
+  const bool fin = (word & kFlagLast) && !A.ku_only;
   for (int s = 0; s < Sg; ++s) {
+    // gather, and this node's finish inputs loaded ahead of the element work
+    float pacc[C], pap[C], pv[C];
+    uint32_t pkc[C];
     if (active) {
 #pragma unroll
-      for (int c = 0; c < C; ++c) su[c * S::ND + p] = gu[((int64_t)s * C + c) * G.G + g];
+      for (int c = 0; c < C; ++c) {
+        const int64_t idx = ((int64_t)s * C + c) * G.G + g;
+        su[c * S::ND + p] = gu[idx];
+        pacc[c] = first ? 0.f : gacc[idx];
+        pap[c] = fin ? gap[idx] : 0.f;
+        pv[c] = fin ? gv[idx] : 0.f;
+        pkc[c] = fin ? gkc[idx] : 0u;
+      }
     }
     __syncthreads();
     if (active) node_flux<DIM, N, PHYS>(X, M, su, sF);
@@ -496,14 +508,11 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         const int64_t idx = ((int64_t)s * C + c) * G.G + g;
-        if (!(word & kFlagLast) || A.ku_only) {
-          if (word & kFlagFirst) gacc[idx] = f[c];
-          else gacc[idx] += f[c];
-        } else {
-          f[c] = (word & kFlagFirst) ? f[c] : gacc[idx] + f[c];
-        }
+        const float t = first ? f[c] : pacc[c] + f[c];
+        if (!fin) gacc[idx] = t;
+        else f[c] = t;
       }
-      if ((word & kFlagLast) && !A.ku_only) {
+      if (fin) {
         // Order2Newmark.cpp:68-115; with the 3-D quirk the list is {x, y, x}
         const bool q3 = (C == 3) && (G.quirks & SEM_QUIRK_NEWMARK3D);
 #pragma unroll
@@ -511,10 +520,9 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
           if (q3 && c == 2) continue;
           const int64_t idx = ((int64_t)s * C + c) * G.G + g;
           float a = mi * f[c];
-          float ap = gap[idx];
-          const uint32_t kc = gkc[idx];
-          float v = gv[idx], kv = comp_v(kc);
-          float u = su[c * S::ND + p], ku = comp_u(kc);
+          float ap = pap[c];
+          float v = pv[c], kv = comp_v(pkc[c]);
+          float u = su[c * S::ND + p], ku = comp_u(pkc[c]);
           const int reps = (q3 && c == 0) ? 2 : 1;
           for (int r = 0; r < reps; ++r) {
             kahan_add(v, kv, G.acl * (a + ap));
