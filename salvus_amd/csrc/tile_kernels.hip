@@ -137,16 +137,25 @@ struct TS {
   static constexpr int NV = DIM == 2 ? 4 : 8;
   static constexpr int NED = NG + NV * KP;                  // floats of element data
   // LDS layout (floats): u lattice | words | lateral fluxes of one chunk |
-  // element-local results | element data | tile source list (4 ints each)
+  // element-local results | element data | tile source list (4 ints each).
+  // With ALIAS the results of all chunks but the last come first and the
+  // last chunk's results overwrite its own lateral fluxes once every column
+  // has read them (one more barrier, less LDS): taken with a single chunk,
+  // and with several where it fits more workgroups in a CU's 160 KiB.
   static constexpr int O_SW = C * NTL;
-  // With a single chunk the element-local results overwrite the lateral
-  // fluxes once every column has read them (one more barrier, less LDS).
-  static constexpr bool ALIAS = NCH == 1;
-  // register budget: 4 waves per SIMD (<= 128 VGPRs)
-  static constexpr int MINW = 4;
-  static constexpr int O_SF = O_SW + NTL;
-  static constexpr int O_SK = ALIAS ? O_SF : O_SF + EC * C * NL * ND;
+  static constexpr int LDS_REST = O_SW + NTL + E * NED + 4 * kTileMaxSrc + 1 + n * n;
+  static constexpr int LDS_SEP = LDS_REST + EC * C * NL * ND + E * C * ND;
+  static constexpr int LDS_ALIAS = LDS_REST + EC * C * NL * ND + (NCH - 1) * EC * C * ND;
+  static constexpr bool ALIAS = NCH == 1 || 40960 / LDS_ALIAS > 40960 / LDS_SEP;
+  static constexpr int O_SF = ALIAS ? O_SW + NTL + (NCH - 1) * EC * C * ND : O_SW + NTL;
+  static constexpr int O_SK = ALIAS ? O_SW + NTL : O_SF + EC * C * NL * ND;
   static constexpr int O_SE = ALIAS ? O_SF + EC * C * NL * ND : O_SK + E * C * ND;
+  // Register budget: 4 waves per SIMD (<= 128 VGPRs); 3-D acoustic tiles
+  // take 6 (<= 80 VGPRs, three workgroups per CU) and load the finish
+  // inputs of three slots per thread ahead instead of all of them.
+  static constexpr int MINW = DIM == 3 && C == 1 ? 6 : 4;
+  static constexpr int KFR = MINW == 4 ? KFMAX : 3;
+  static constexpr int KF = NIT < KFR ? NIT : KFR;
   static constexpr int O_SS = O_SE + E * NED;
   static constexpr int O_SD = O_SS + 4 * kTileMaxSrc + 1;
   static constexpr int LDS = O_SD + n * n;
@@ -467,7 +476,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
             for (int k = 0; k < n; ++k) f[c][k] = fmaf(ld[c * ND + pl + T::ES * k], R, f[c][k]);
         }
       }
-      if constexpr (T::ALIAS) __syncthreads();  // lateral fluxes read before sK overwrites them
+      if (T::ALIAS && ch == T::NCH - 1) __syncthreads();  // lateral fluxes read before sK overwrites them
       if (act) {
 #pragma unroll
         for (int c = 0; c < C; ++c)
@@ -501,7 +510,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
         in.kc[c] = fin ? gkc[idx] : 0u;
       }
     };
-    constexpr int KF = T::NIT < T::KFMAX ? T::NIT : T::KFMAX;
+    constexpr int KF = T::KF;
     SlotIn pre[KF];
 #pragma unroll
     for (int r = 0; r < KF; ++r) load_slot(r, pre[r]);
