@@ -84,10 +84,10 @@ int sem_mesh_destroy(sem_mesh *mesh);
  *                shot-step, elements per tile (padded), 0} */
 int sem_mesh_info(const sem_mesh *mesh, int64_t *info);
 /* Tile-shape autotuner (elements per workgroup of the tile kernel): for
- * every compiled tile shape of the mesh's order and physics, times `reps`
+ * every compiled tile shape of the mesh's order and physics, times 2 x `reps`
  * full batched steps of one group of `shots` shots (the batch size the mesh
- * will serve; the best shape depends on it) twice and keeps the shape with
- * the fastest minimum; *shape = Tx*10000 + Ty*100 + Tz (0: the mesh takes
+ * will serve; the best shape depends on it) one by one and keeps the shape
+ * with the fastest step; *shape = Tx*10000 + Ty*100 + Tz (0: the mesh takes
  * the element kernel).  No reference counterpart.  Call before the mesh
  * joins a batch. */
 int sem_mesh_autotune(sem_mesh *mesh, int reps, int shots, int64_t *shape);

@@ -168,18 +168,25 @@ __device__ __forceinline__ void kahan(float &s, float &c, float x) {
   s = t;
 }
 
-// P1 vertex weight at (a, b[, t]) with the reference's vertex order
-// (QuadP1.h:98-108, HexP1.cpp:242-298).  In 2-D the column coordinate t is eta.
+// P1 vertices (QuadP1.h:98-108, HexP1.cpp:242-298): the multilinear weight
+// of vertex v at (a, b[, t]) is prod (1 + s_i x_i) / 2^DIM with the vertex's
+// signs s.  In 2-D the column coordinate t is eta.  Along the column the
+// weights are linear in t, so the material is fixed by its values on the two
+// end faces t = -1 and t = +1, where only that face's vertices take part.
 template <int DIM>
-__device__ __forceinline__ float p1w(int v, float a, float b, float t) {
+__device__ __forceinline__ int p1_tsign(int v) {
+  constexpr signed char ts[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+  return DIM == 2 ? (v >= 2 ? 1 : -1) : ts[v];
+}
+// weight of vertex v on its own end face (t = its sign)
+template <int DIM>
+__device__ __forceinline__ float p1_face(int v, float a, float b) {
   if (DIM == 2) {
     const float sa = (v == 1 || v == 2) ? 1.f : -1.f;
-    const float st = (v >= 2) ? 1.f : -1.f;
-    return 0.25f * (1.f + sa * a) * (1.f + st * t);
+    return 0.5f * (1.f + sa * a);
   }
-  constexpr signed char hs[8][3] = {{-1, -1, -1}, {-1, 1, -1}, {1, 1, -1}, {1, -1, -1},
-                                    {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
-  return 0.125f * (1.f + hs[v][0] * a) * (1.f + hs[v][1] * b) * (1.f + hs[v][2] * t);
+  constexpr signed char hs[8][2] = {{-1, -1}, {-1, 1}, {1, 1}, {1, -1}, {-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+  return 0.25f * (1.f + hs[v][0] * a) * (1.f + hs[v][1] * b);
 }
 
 template <int DIM, int N, int PHYS, int TX, int TY, int TZ>
@@ -317,25 +324,23 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
         for (int c = 0; c < C; ++c)
 #pragma unroll
           for (int k = 0; k < n; ++k) uc[c][k] = su[c * T::NTL + cb + k * T::CS];
-        // vertex material is trilinear: linear along the column
+        // vertex material at t: mA + mB t, from its end-face values
         float mA[T::KP], mB[T::KP];
         {
-          float w0[T::NV], w1[T::NV];
+          float wf[T::NV];
 #pragma unroll
-          for (int v = 0; v < T::NV; ++v) {
-            w0[v] = p1w<DIM>(v, xa, xb, 0.f);
-            w1[v] = p1w<DIM>(v, xa, xb, 1.f);
-          }
+          for (int v = 0; v < T::NV; ++v) wf[v] = 0.5f * p1_face<DIM>(v, xa, xb);
 #pragma unroll
           for (int q = 0; q < T::KP; ++q) {
-            float a0 = 0.f, a1 = 0.f;
+            float lo = 0.f, hi = 0.f;
 #pragma unroll
             for (int v = 0; v < T::NV; ++v) {
-              a0 = fmaf(w0[v], ed[T::NG + v * T::KP + q], a0);
-              a1 = fmaf(w1[v], ed[T::NG + v * T::KP + q], a1);
+              const float P = ed[T::NG + v * T::KP + q];
+              if (p1_tsign<DIM>(v) < 0) lo = fmaf(wf[v], P, lo);
+              else hi = fmaf(wf[v], P, hi);
             }
-            mA[q] = a0;
-            mB[q] = a1 - a0;
+            mA[q] = hi + lo;
+            mB[q] = hi - lo;
           }
         }
 #pragma unroll
