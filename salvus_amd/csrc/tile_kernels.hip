@@ -342,6 +342,14 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
             mA[q] = hi + lo;
             mB[q] = hi - lo;
           }
+          if constexpr (PHYS != SEM_ACOUSTIC && DIM == 3) {
+            // every Elastic3D coefficient is proportional to rho: scaling rho
+            // by w_i w_j det J (and by w_k below) puts the quadrature weight
+            // into the stress, so the flux needs no scaling of its own
+            const float sw = wl * ed[DIM * DIM];
+            mA[0] *= sw;
+            mB[0] *= sw;
+          }
         }
 #pragma unroll
         for (int k = 0; k < n; ++k) {
@@ -390,6 +398,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
               float q6[6];
 #pragma unroll
               for (int q = 0; q < 6; ++q) q6[q] = fmaf(mB[q], tk, mA[q]);
+              q6[0] *= wk;
               const float c11 = q6[0] * q6[2] * q6[2], c33 = q6[0] * q6[1] * q6[1];
               const float c44 = q6[0] * q6[3] * q6[3], c66 = q6[0] * q6[4] * q6[4];
               const float c13 = q6[5] * (c11 - 2.f * c44);
@@ -430,7 +439,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ>::BT))
                 float v = 0.f;
 #pragma unroll
                 for (int b = 0; b < DIM; ++b) v = fmaf(Ji[b * DIM + a], sig[c][b], v);
-                F[c < C ? c : 0][a] = wdet * v;
+                F[c < C ? c : 0][a] = DIM == 3 ? v : wdet * v;
               }
           }
           const int p = pl + T::ES * k;
