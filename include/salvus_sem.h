@@ -171,6 +171,15 @@ int sem_receivers_write_h5(const char *path, int append, int nrec, const char *c
  * matrix: component k's series is out[k * nsteps + n]); the dataset must hold
  * exactly nsteps * ncomp values. */
 int sem_source_read_h5(const char *path, const char *name, int64_t nsteps, int ncomp, double *out);
+/* Problem::saveSolution (Problem.cpp:217-237; the driver calls it after
+ * every step, Main.cpp:55-62, with the time after the step): appends one
+ * movie frame to `path` (append = 0 creates / truncates it): field k's n
+ * values (values[k * n + i], global DOF order) to the extendible FP64
+ * dataset /fields/<names[k]> [frames][n], and `time` to /time [frames].
+ * Every dataset of a frame must hold the same number of frames before it is
+ * appended (else -1 and nothing is written).  Names hold no '/'. */
+int sem_movie_write_h5(const char *path, int append, int nfields, const char *const *names,
+                       int64_t n, const double *values, double time);
 /* Launch configuration chosen: cfg[0..3] = {elems/block, shots/group,
  * graph block steps, groups}. */
 int sem_batch_config(const sem_batch *batch, int64_t *cfg);

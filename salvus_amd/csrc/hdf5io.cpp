@@ -9,11 +9,20 @@
 //              column-major matrix (component k's series is the k-th run of
 //              nsteps values); fire() returns row `time_idx`
 //              (SourceHdf5.cpp:31-36).
+//   movies     Problem::saveSolution (Problem.cpp:217-237), called after
+//              every step by the driver (Main.cpp:55-62): each named field
+//              of the frame goes to an extendible FP64 dataset
+//              /fields/<name> [frames][n], and the frame's time (the time
+//              after the step, the value DMSetOutputSequenceNumber records)
+//              to /time [frames].  This follows PETSc's time-stepped
+//              VecView layout for DMPlex vectors in outline; PETSc itself is
+//              not available to check it against.
 // libhdf5 is opened at run time (SEM_HDF5_LIB, else the usual names), so
 // the library needs it only when these entry points are called.
 #include <dlfcn.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <string>
@@ -30,6 +39,8 @@ using hsize_t = unsigned long long;
 using hssize_t = long long;
 constexpr unsigned kAccRdonly = 0x0000u, kAccRdwr = 0x0001u, kAccTrunc = 0x0002u;
 constexpr hid_t kDefault = 0;  // H5P_DEFAULT, H5S_ALL, H5E_DEFAULT
+constexpr hsize_t kUnlimited = ~0ull;  // H5S_UNLIMITED
+constexpr int kSelectSet = 0;          // H5S_SELECT_SET
 
 struct H5 {
   void *lib = nullptr;
@@ -50,7 +61,14 @@ struct H5 {
   herr_t (*dwrite)(hid_t, hid_t, hid_t, hid_t, hid_t, const void *) = nullptr;
   herr_t (*dread)(hid_t, hid_t, hid_t, hid_t, hid_t, void *) = nullptr;
   herr_t (*dclose)(hid_t) = nullptr;
-  const hid_t *native_float = nullptr, *native_double = nullptr;
+  hid_t (*pcreate)(hid_t) = nullptr;
+  herr_t (*pset_chunk)(hid_t, int, const hsize_t *) = nullptr;
+  herr_t (*pclose)(hid_t) = nullptr;
+  herr_t (*dset_extent)(hid_t, const hsize_t *) = nullptr;
+  herr_t (*sselect_hyperslab)(hid_t, int, const hsize_t *, const hsize_t *, const hsize_t *,
+                              const hsize_t *) = nullptr;
+  int (*sget_dims)(hid_t, hsize_t *, hsize_t *) = nullptr;
+  const hid_t *native_float = nullptr, *native_double = nullptr, *dcpl_class = nullptr;
 };
 
 template <class F>
@@ -90,6 +108,11 @@ const H5 *hdf5() {
             sym(lib, "H5Dcreate2", h.dcreate) && sym(lib, "H5Dopen2", h.dopen) &&
             sym(lib, "H5Dget_space", h.dspace) && sym(lib, "H5Dwrite", h.dwrite) &&
             sym(lib, "H5Dread", h.dread) && sym(lib, "H5Dclose", h.dclose) &&
+            sym(lib, "H5Pcreate", h.pcreate) && sym(lib, "H5Pset_chunk", h.pset_chunk) &&
+            sym(lib, "H5Pclose", h.pclose) && sym(lib, "H5Dset_extent", h.dset_extent) &&
+            sym(lib, "H5Sselect_hyperslab", h.sselect_hyperslab) &&
+            sym(lib, "H5Sget_simple_extent_dims", h.sget_dims) &&
+            sym(lib, "H5P_CLS_DATASET_CREATE_ID_g", h.dcpl_class) &&
             sym(lib, "H5T_NATIVE_FLOAT_g", h.native_float) &&
             sym(lib, "H5T_NATIVE_DOUBLE_g", h.native_double);
   if (!ok || h.open() < 0) {
@@ -107,6 +130,56 @@ bool plain_name(const char *s) {
   for (; *s; ++s)
     if (*s == '/') return false;
   return true;
+}
+
+// Rows already in the extendible dataset `name` ([rows][width], or [rows]
+// when width == 0), -1 if it does not exist, -2 on a shape mismatch.
+int64_t rows_of(const H5 *h, hid_t f, const std::string &name, int64_t width) {
+  if (h->lexists(f, name.c_str(), kDefault) <= 0) return -1;
+  const hid_t ds = h->dopen(f, name.c_str(), kDefault);
+  if (ds < 0) return -2;
+  const hid_t sp = h->dspace(ds);
+  hsize_t dims[2] = {0, 0};
+  const int rank = sp < 0 ? -1 : h->sget_dims(sp, dims, nullptr);
+  if (sp >= 0) h->sclose(sp);
+  h->dclose(ds);
+  if (rank != (width > 0 ? 2 : 1) || (width > 0 && dims[1] != (hsize_t)width)) return -2;
+  return (int64_t)dims[0];
+}
+
+// Appends one row (width doubles, or one double when width == 0) to the
+// extendible dataset `name`, creating it ([0][width], chunks of one row)
+// when absent.
+bool append_row(const H5 *h, hid_t f, const std::string &name, int64_t width, const double *row) {
+  const int rank = width > 0 ? 2 : 1;
+  const hsize_t w = width > 0 ? (hsize_t)width : 1;
+  int64_t rows = rows_of(h, f, name, width);
+  if (rows == -2) return false;
+  hid_t ds;
+  if (rows < 0) {
+    const hsize_t dims[2] = {0, w}, maxd[2] = {kUnlimited, w};
+    const hsize_t chunk[2] = {width > 0 ? 1ull : 256ull, w};
+    const hid_t sp = h->screate_simple(rank, dims, maxd);
+    const hid_t pl = h->pcreate(*h->dcpl_class);
+    const bool ok = sp >= 0 && pl >= 0 && h->pset_chunk(pl, rank, chunk) >= 0;
+    ds = ok ? h->dcreate(f, name.c_str(), *h->native_double, sp, kDefault, pl, kDefault) : -1;
+    if (pl >= 0) h->pclose(pl);
+    if (sp >= 0) h->sclose(sp);
+    rows = 0;
+  } else {
+    ds = h->dopen(f, name.c_str(), kDefault);
+  }
+  if (ds < 0) return false;
+  const hsize_t ext[2] = {(hsize_t)rows + 1, w}, start[2] = {(hsize_t)rows, 0}, count[2] = {1, w};
+  bool ok = h->dset_extent(ds, ext) >= 0;
+  const hid_t fs = ok ? h->dspace(ds) : -1;
+  const hid_t ms = fs >= 0 ? h->screate_simple(1, &w, nullptr) : -1;
+  ok = ok && ms >= 0 && h->sselect_hyperslab(fs, kSelectSet, start, nullptr, count, nullptr) >= 0 &&
+       h->dwrite(ds, *h->native_double, ms, fs, kDefault, row) >= 0;
+  if (ms >= 0) h->sclose(ms);
+  if (fs >= 0) h->sclose(fs);
+  h->dclose(ds);
+  return ok;
 }
 
 }  // namespace
@@ -188,6 +261,43 @@ int sem_source_read_h5(const char *path, const char *name, int64_t nsteps, int n
     h->dclose(ds);
   }
   h->fclose(f);
+  return rc;
+}
+
+int sem_movie_write_h5(const char *path, int append, int nfields, const char *const *names,
+                       int64_t n, const double *values, double time) {
+  if (!path || nfields < 1 || !names || n < 1 || !values) {
+    sem::set_error("movie_write_h5: bad arguments");
+    return -1;
+  }
+  for (int k = 0; k < nfields; ++k)
+    if (!plain_name(names[k])) { sem::set_error("field names must be non-empty and contain no '/'"); return -1; }
+  const H5 *h = hdf5();
+  if (!h) return -2;
+  hid_t f = (append && access(path, F_OK) == 0) ? h->fopen(path, kAccRdwr, kDefault)
+                                                : h->fcreate(path, kAccTrunc, kDefault, kDefault);
+  if (f < 0) { sem::set_error(std::string("cannot open ") + path); return -2; }
+  int rc = 0;
+  // every dataset of the frame must hold the same number of frames
+  const int64_t frames = std::max<int64_t>(rows_of(h, f, "time", 0), 0);
+  if (rows_of(h, f, "time", 0) == -2) rc = -1;
+  for (int k = 0; k < nfields && !rc; ++k) {
+    const int64_t r = rows_of(h, f, std::string("fields/") + names[k], n);
+    if (r == -2 || (r >= 0 && r != frames) || (r < 0 && frames > 0)) rc = -1;
+  }
+  if (rc) sem::set_error("movie file datasets do not match this frame (field set, length or frame count)");
+  if (!rc && h->lexists(f, "fields", kDefault) <= 0) {
+    const hid_t g = h->gcreate(f, "fields", kDefault, kDefault, kDefault);
+    if (g < 0) { sem::set_error("H5Gcreate failed"); rc = -2; }
+    else h->gclose(g);
+  }
+  for (int k = 0; k < nfields && !rc; ++k)
+    if (!append_row(h, f, std::string("fields/") + names[k], n, values + (size_t)k * n)) {
+      sem::set_error(std::string("writing /fields/") + names[k] + " failed");
+      rc = -2;
+    }
+  if (!rc && !append_row(h, f, "time", 0, &time)) { sem::set_error("writing /time failed"); rc = -2; }
+  if (h->fclose(f) < 0 && !rc) { sem::set_error("H5Fclose failed"); rc = -2; }
   return rc;
 }
 

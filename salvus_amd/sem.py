@@ -39,7 +39,7 @@ EXPORTS = (
     "sem_reference_step_count", "sem_server_create", "sem_server_destroy", "sem_server_clock",
     "sem_server_submit", "sem_server_pump", "sem_server_request", "sem_server_result",
     "sem_mesh_autotune", "sem_shot_add_file_source", "sem_receivers_write_h5",
-    "sem_source_read_h5",
+    "sem_source_read_h5", "sem_movie_write_h5",
 )
 SOURCE_DOUBLES = 19
 
@@ -109,6 +109,8 @@ def lib():
                                              C.POINTER(C.c_char_p), C.c_int,
                                              C.POINTER(C.c_char_p), C.c_int64, _F]
         L.sem_source_read_h5.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, C.c_int, _D]
+        L.sem_movie_write_h5.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_char_p),
+                                         C.c_int64, _D, C.c_double]
         _lib = L
     return _lib
 
@@ -176,6 +178,26 @@ def read_source_h5(path, name, nsteps, ncomp):
     out = np.zeros(int(nsteps) * int(ncomp))
     _check(lib().sem_source_read_h5(str(path).encode(), name.encode(), nsteps, ncomp, _d(out)))
     return out.reshape(ncomp, nsteps).T.copy()
+
+
+def movie_field_names(c):
+    """The reference's global field names of a physics with c components
+    (Order2Newmark fields u, v, a, a_; Elastic*D with x, y[, z] suffixes), in
+    the order (field, component) they map to: a and a_ are the same values
+    after a step (takeTimeStep sets a_ = a)."""
+    sfx = ("",) if c == 1 else ("x", "y", "z")[:c]
+    return {b + s + t: (FIELD["a_" if b == "a" else b], k)
+            for b, t in (("u", ""), ("v", ""), ("a", ""), ("a", "_")) for k, s in enumerate(sfx)}
+
+
+def write_movie_frame(path, fields, time, append=True):
+    """Problem::saveSolution layout: appends one frame, fields = {name:
+    values over the global DOFs}, to /fields/<name> [frames][n] (FP64) and
+    the time to /time."""
+    names = list(fields)
+    vals = np.ascontiguousarray(np.stack([np.asarray(fields[k], dtype=np.float64).ravel() for k in names]))
+    _check(lib().sem_movie_write_h5(str(path).encode(), int(bool(append)), len(names), _cstrings(names),
+                                    vals.shape[1], _d(vals), float(time)))
 
 
 def topology(dim, order, cells, nvert):
@@ -365,6 +387,25 @@ class Batch:
         """Seismograms of one shot in the reference's receiver file format."""
         write_receivers_h5(path, names, self.seismograms(shot), append=append)
 
+    def movie_frame(self, shot, path, names, append=True):
+        """One movie frame of the named global fields of a shot (the
+        reference's names: u, v, a, a_ or ux, uy, ..., ax_, ...) at the
+        shot's current time (Problem::saveSolution, Problem.cpp:217-237)."""
+        m = self.shots[shot][0]
+        known = movie_field_names(m.c)
+        for f in names:
+            if f not in known:
+                raise KeyError("You are attempting to save field %s which is not registered. "
+                               "Registered fields are { %s }." % (f, " ".join(known)))
+        got = {}
+        frame = {}
+        for f in names:
+            which, k = known[f]
+            if which not in got:
+                got[which] = self.get_field(shot, [w for w, i in FIELD.items() if i == which][0])
+            frame[f] = got[which][k]
+        write_movie_frame(path, frame, self.progress(shot)[1], append)
+
     def config(self):
         c = np.zeros(4, dtype=np.int64)
         _check(lib().sem_batch_config(self._h, _l(c)))
@@ -420,6 +461,13 @@ class Order2Newmark:
 
     def time(self):
         return self.batch.progress(self.shot)[1]
+
+    def saveSolution(self, path, save_fields, append=True):
+        """Problem::saveSolution (Problem.cpp:217-237): one movie frame of
+        save_fields at the current time; the driver calls it after every
+        step (Main.cpp:55-62)."""
+        self.initializeGlobalDofs()
+        self.batch.movie_frame(self.shot, path, save_fields, append)
 
 
 def source_record(src):

@@ -88,3 +88,32 @@ def test_hdf5_source_to_hdf5_receivers(tmp_path):
             np.testing.assert_array_equal(read_dataset(rec_file, "/%s/%s" % (name, field))[0],
                                           seis[:, i, c])
     assert np.abs(seis).max() > 0
+
+
+@pytest.mark.skipif(H5 is None, reason="libhdf5 not found on this host")
+def test_movie_every_step(tmp_path):
+    """Problem::saveSolution after every step (Main.cpp:55-62): the frames in
+    the file are the fields the shot held after each step, at the time after
+    that step."""
+    P = pb.make(2, 1, 8, order=4, seed=81, nsteps=6)
+    M = sem.Mesh(2, 4, 1, P["cells"], P["coords"], P["par"])
+    prob = sem.Order2Newmark(M, P["dt"], P["nsteps"])
+    s = P["src"]
+    prob.add_source(s["loc"], s["amp"], s["f0"], s["t0"], s["dir"])
+    path = tmp_path / "movie.h5"
+    want = {"ux": [], "vy": [], "ay_": []}
+    for step in range(4):
+        prob.takeTimeStep(1)
+        prob.saveSolution(path, list(want), append=step > 0)
+        u, v, a = prob.get_field("u"), prob.get_field("v"), prob.get_field("a_")
+        want["ux"].append(u[0])
+        want["vy"].append(v[1])
+        want["ay_"].append(a[1])
+    for name, frames in want.items():
+        got = read_dataset(path, "/fields/" + name, np.float64)[0]
+        np.testing.assert_array_equal(got, np.stack(frames))
+    assert np.abs(np.stack(want["ux"])).max() > 0
+    t = read_dataset(path, "/time", np.float64)[0]
+    np.testing.assert_allclose(t, P["dt"] * np.arange(1, 5), rtol=1e-12)
+    with pytest.raises(KeyError, match="not registered"):
+        prob.saveSolution(path, ["uz"])

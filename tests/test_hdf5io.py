@@ -1,6 +1,8 @@
 """HDF5 files either side of the time loop (host only, no GPU): receiver
-files in the ReceiverHdf5::write layout (ReceiverHdf5.cpp:31-87) and source
-files read the way SourceHdf5::loadData reads them (SourceHdf5.cpp:40-50).
+files in the ReceiverHdf5::write layout (ReceiverHdf5.cpp:31-87), source
+files read the way SourceHdf5::loadData reads them (SourceHdf5.cpp:40-50)
+and movie frames appended the way Problem::saveSolution is called
+(Problem.cpp:217-237, after every step).
 The files are checked with libhdf5 called directly from the test, not
 through the library under test."""
 import ctypes as C
@@ -66,7 +68,7 @@ H5 = _libhdf5()
 pytestmark = pytest.mark.skipif(H5 is None, reason="libhdf5 not found on this host")
 
 
-def read_dataset(path, name):
+def read_dataset(path, name, dtype=np.float32):
     f = H5.H5Fopen(str(path).encode(), 0, DEFAULT)
     assert f >= 0
     d = H5.H5Dopen2(f, name.encode(), DEFAULT)
@@ -77,8 +79,9 @@ def read_dataset(path, name):
     H5.H5Sget_simple_extent_dims(sp, dims, None)
     ty = H5.H5Dget_type(d)
     size, cls = H5.H5Tget_size(ty), H5.H5Tget_class(ty)
-    out = np.zeros([dims[i] for i in range(nd)], dtype=np.float32)
-    assert H5.H5Dread(d, H5.float_t, DEFAULT, DEFAULT, DEFAULT, out.ctypes.data) >= 0
+    out = np.zeros([dims[i] for i in range(nd)], dtype=dtype)
+    mem_t = H5.double_t if dtype == np.float64 else H5.float_t
+    assert H5.H5Dread(d, mem_t, DEFAULT, DEFAULT, DEFAULT, out.ctypes.data) >= 0
     H5.H5Tclose(ty)
     H5.H5Sclose(sp)
     H5.H5Dclose(d)
@@ -155,3 +158,37 @@ def test_source_file_column_major(tmp_path):
         sem.read_source_h5(path, "missing", nsteps, ncomp)
     with pytest.raises(sem.SemError, match="cannot open"):
         sem.read_source_h5(tmp_path / "nope.h5", "src0", nsteps, ncomp)
+
+
+def test_movie_frames(tmp_path):
+    rng = np.random.default_rng(5)
+    path = tmp_path / "movie.h5"
+    frames = [{"ux": rng.standard_normal(11), "uy": rng.standard_normal(11)} for _ in range(3)]
+    times = [0.1, 0.2, 0.30000000000000004]
+    for i, (fr, t) in enumerate(zip(frames, times)):
+        sem.write_movie_frame(path, fr, t, append=i > 0)
+    for name in ("ux", "uy"):
+        arr, size, cls = read_dataset(path, "/fields/" + name, np.float64)
+        assert arr.shape == (3, 11) and size == 8 and cls == 1  # H5T_FLOAT, 8 bytes
+        np.testing.assert_array_equal(arr, np.stack([fr[name] for fr in frames]))
+    t, size, _ = read_dataset(path, "/time", np.float64)
+    assert size == 8
+    np.testing.assert_array_equal(t, times)
+    # a frame must match what the file holds: same length, every field present
+    with pytest.raises(sem.SemError, match="do not match this frame"):
+        sem.write_movie_frame(path, {"ux": np.zeros(12)}, 0.4)
+    with pytest.raises(sem.SemError, match="do not match this frame"):
+        sem.write_movie_frame(path, {"vx": np.zeros(11)}, 0.4)
+    assert read_dataset(path, "/time", np.float64)[0].shape == (3,)
+    with pytest.raises(sem.SemError, match="no '/'"):
+        sem.write_movie_frame(path, {"a/b": np.zeros(11)}, 0.4)
+    sem.write_movie_frame(path, {"u": np.arange(4.0)}, 1.5, append=False)  # a fresh file
+    assert not exists(path, "fields/ux")
+    np.testing.assert_array_equal(read_dataset(path, "/fields/u", np.float64)[0], [np.arange(4.0)])
+
+
+def test_movie_field_names():
+    assert set(sem.movie_field_names(1)) == {"u", "v", "a", "a_"}
+    names = sem.movie_field_names(3)
+    assert set(names) == {p + s + q for p in "uva" for s in "xyz" for q in ("",)} | {"ax_", "ay_", "az_"}
+    assert names["vy"] == (sem.FIELD["v"], 1) and names["az_"] == names["az"] == (sem.FIELD["a_"], 2)
