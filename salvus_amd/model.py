@@ -15,9 +15,9 @@ Elastic2D.cpp:29-36, Elastic3D.cpp:33-40):
   Elastic2D        RHO, C11, C13, C33, C55
   Elastic3D        RHO, VPV, VPH, VSV, VSH, ETA
 
-`vertex_parameters` returns the E x nv x npar array that `sem.Mesh` takes.
-Reading the Exodus file itself is outside this module: pass the model's
-element centres and its elemental variables as arrays.
+`vertex_parameters` returns the E x nv x npar array that `sem.Mesh` takes;
+`exodus.read_model` supplies the model's element centres and elemental
+variables from an Exodus file.
 """
 import numpy as np
 from scipy.spatial import cKDTree

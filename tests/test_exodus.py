@@ -1,0 +1,145 @@
+"""Exodus II files (host only): meshes read as Mesh::read takes them
+(DMPlexCreateExodusFromFile: all blocks, reference vertex order) and models
+as ExodusModel::read holds them (first block, elemental variables of block id
+1 at time step 1).  The files are written with scipy's netCDF writer in the
+Exodus II layout, independently of the reader under test; the hexahedron
+vertex permutation is pinned to the reference's vertex table
+(test_HexP1.cpp:21-48)."""
+import numpy as np
+import pytest
+from scipy.io import netcdf_file
+
+from salvus_amd import exodus, meshgen, model, sem
+
+HEX_REF = np.array([[-1, -1, -1], [-1, 1, -1], [1, 1, -1], [1, -1, -1],
+                    [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], float)
+# Exodus HEX8 canonical node order: bottom face counter-clockwise seen from +z
+EXO_HEX8 = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                     [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], float)
+
+
+def _chars(strings, width=33):
+    a = np.zeros((len(strings), width), dtype="S1")
+    for i, s in enumerate(strings):
+        for j, ch in enumerate(s.encode()):
+            a[i, j] = bytes([ch])
+    return a
+
+
+def write_exodus(path, xyz, blocks, elem_vars=None, side_sets=(), large=True, block_ids=None):
+    """blocks: list of (1-based connectivity, elem_type); elem_vars: {name:
+    values over block 1}; side_sets: [(name, 1-based elements, sides)]."""
+    f = netcdf_file(str(path), "w", version=2)
+    n, dim = xyz.shape
+    f.createDimension("time_step", None)  # scipy: the record dimension comes first
+    f.createDimension("len_name", 33)
+    f.createDimension("num_dim", dim)
+    f.createDimension("num_nodes", n)
+    f.createDimension("num_elem", sum(len(c) for c, _ in blocks))
+    f.createDimension("num_el_blk", len(blocks))
+    t = f.createVariable("time_whole", "d", ("time_step",))
+    t[0] = 0.0
+    if large:
+        for k, a in enumerate("xyz"[:dim]):
+            f.createVariable("coord" + a, "d", ("num_nodes",))[:] = xyz[:, k]
+    else:
+        f.createVariable("coord", "d", ("num_dim", "num_nodes"))[:] = xyz.T
+    ids = block_ids or list(range(1, len(blocks) + 1))
+    prop = f.createVariable("eb_prop1", "i", ("num_el_blk",))
+    prop[:] = ids
+    prop.name = b"ID"
+    for b, (conn, etype) in enumerate(blocks, start=1):
+        f.createDimension("num_el_in_blk%d" % b, len(conn))
+        f.createDimension("num_nod_per_el%d" % b, conn.shape[1])
+        v = f.createVariable("connect%d" % b, "i", ("num_el_in_blk%d" % b, "num_nod_per_el%d" % b))
+        v[:] = conn
+        v.elem_type = etype.encode()
+    if elem_vars:
+        names = list(elem_vars)
+        f.createDimension("num_elem_var", len(names))
+        f.createVariable("name_elem_var", "c", ("num_elem_var", "len_name"))[:] = _chars(names)
+        for i, name in enumerate(names, start=1):
+            v = f.createVariable("vals_elem_var%deb1" % i, "d", ("time_step", "num_el_in_blk1"))
+            v[0] = elem_vars[name]
+    if side_sets:
+        f.createDimension("num_side_sets", len(side_sets))
+        f.createVariable("ss_names", "c", ("num_side_sets", "len_name"))[:] = _chars([s[0] for s in side_sets])
+        for s, (_, el, sd) in enumerate(side_sets, start=1):
+            f.createDimension("num_side_ss%d" % s, len(el))
+            f.createVariable("elem_ss%d" % s, "i", ("num_side_ss%d" % s,))[:] = el
+            f.createVariable("side_ss%d" % s, "i", ("num_side_ss%d" % s,))[:] = sd
+    f.close()
+
+
+def test_hex8_order_becomes_the_reference_vertex_order(tmp_path):
+    path = tmp_path / "cube.e"
+    write_exodus(path, EXO_HEX8, [(np.arange(1, 9)[None, :], "HEX8")])
+    dim, cells, xyz, ss = exodus.read_mesh(path)
+    assert dim == 3 and cells.shape == (1, 8) and ss == []
+    np.testing.assert_array_equal(xyz[cells[0]], HEX_REF)
+
+
+def test_quad_mesh_blocks_side_sets_and_model(tmp_path):
+    cells, coords = meshgen.box_quad(3, 2, 30.0, 20.0)
+    E = len(cells)
+    path = tmp_path / "quads.e"
+    rng = np.random.default_rng(2)
+    vp = {"VP_%d" % v: rng.uniform(1000, 2000, E // 2) for v in range(4)}
+    write_exodus(path, coords, [(cells[: E // 2] + 1, "QUAD4"), (cells[E // 2:] + 1, "QUAD")],
+                 elem_vars=vp, side_sets=[("absorbing_boundary", [1, 4], [4, 1])], large=False)
+    dim, got_cells, xyz, ss = exodus.read_mesh(path)
+    assert dim == 2
+    np.testing.assert_array_equal(got_cells, cells)  # blocks in order, QUAD4 order kept
+    np.testing.assert_array_equal(xyz, coords)
+    assert len(ss) == 1 and ss[0][0] == "absorbing_boundary"
+    np.testing.assert_array_equal(ss[0][1], [0, 3])
+    np.testing.assert_array_equal(ss[0][2], [4, 1])
+    # the model: first block only, as ExodusModel holds it
+    mdim, centres, variables = exodus.read_model(path)
+    assert mdim == 2 and set(variables) == set(vp)
+    np.testing.assert_allclose(centres, model.element_centres(cells[: E // 2], coords))
+    for k in vp:
+        np.testing.assert_array_equal(variables[k], vp[k])
+    # served to the first block's own elements, the variables come back
+    par = model.vertex_parameters(2, 0, cells[: E // 2], coords, centres, variables)
+    np.testing.assert_array_equal(par[:, :, 0], np.stack([vp["VP_%d" % v] for v in range(4)], axis=1))
+
+
+def test_hex_mesh_feeds_the_topology(tmp_path):
+    cells, coords = meshgen.box_hex(2, 2, 1, 20.0, 20.0, 10.0)
+    exo = cells[:, [0, 3, 2, 1, 4, 5, 6, 7]] + 1  # the reference's order back to Exodus HEX8
+    path = tmp_path / "hexes.e"
+    names = ["RHO", "VPV", "VPH", "VSV", "VSH", "ETA"]
+    ev = {"%s_%d" % (n, v): np.full(len(cells), 1.0 + q + 0.1 * v) for q, n in enumerate(names)
+          for v in range(8)}
+    write_exodus(path, coords, [(exo, "HEX8")], elem_vars=ev)
+    dim, got, xyz, _ = exodus.read_mesh(path)
+    np.testing.assert_array_equal(got, cells)
+    conn, colour, info = sem.topology(3, 4, got, len(xyz))
+    assert conn.shape == (len(cells), 125)
+    _, centres, variables = exodus.read_model(path)
+    par = model.vertex_parameters(3, 1, got, xyz, centres, variables)
+    assert par.shape == (len(cells), 8, 6) and par[0, 7, 5] == pytest.approx(6.7)
+
+
+def test_errors(tmp_path):
+    h5 = tmp_path / "nc4.e"
+    h5.write_bytes(b"\x89HDF\r\n\x1a\n" + bytes(64))
+    with pytest.raises(exodus.ExodusError, match="netCDF-4"):
+        exodus.read_mesh(h5)
+    junk = tmp_path / "junk.e"
+    junk.write_bytes(b"not a mesh")
+    with pytest.raises(exodus.ExodusError, match="not a classic"):
+        exodus.read_model(junk)
+    tri = tmp_path / "tri.e"
+    write_exodus(tri, np.array([[0.0, 0], [1, 0], [0, 1]]), [(np.array([[1, 2, 3]]), "TRI3")])
+    with pytest.raises(exodus.ExodusError, match="QUAD4"):
+        exodus.read_mesh(tri)
+    other = tmp_path / "ids.e"
+    cells, coords = meshgen.box_quad(1, 1)
+    write_exodus(other, coords, [(cells + 1, "QUAD4")], elem_vars={"VP_0": np.ones(1)}, block_ids=[7])
+    with pytest.raises(exodus.ExodusError, match="id 1"):
+        exodus.read_model(other)
+    plain = tmp_path / "novars.e"
+    write_exodus(plain, coords, [(cells + 1, "QUAD4")])
+    assert exodus.read_model(plain)[2] == {}
