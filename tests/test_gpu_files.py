@@ -117,3 +117,28 @@ def test_movie_every_step(tmp_path):
     np.testing.assert_allclose(t, P["dt"] * np.arange(1, 5), rtol=1e-12)
     with pytest.raises(KeyError, match="not registered"):
         prob.saveSolution(path, ["uz"])
+
+
+@pytest.mark.parametrize("dim,physics,distorted", [(3, 1, False), (2, 0, True)])
+def test_exodus_mesh_and_model_run(tmp_path, dim, physics, distorted):
+    """A mesh and model written as an Exodus II file, read back through
+    exodus.read_mesh / read_model and model.vertex_parameters, run exactly
+    as the arrays they were written from."""
+    from salvus_amd import exodus, model
+    from tests.test_exodus import write_exodus
+    P = pb.make(dim, physics, 5 if dim == 3 else 10, order=4, seed=90 + dim, distorted=distorted,
+                nsteps=60)
+    cells = P["cells"]
+    exo = cells[:, [0, 3, 2, 1, 4, 5, 6, 7]] if dim == 3 else cells
+    names = model.PARAMETERS[(dim, physics)]
+    ev = {"%s_%d" % (n, v): P["par"][:, v, q] for q, n in enumerate(names) for v in range(cells.shape[1])}
+    path = tmp_path / "model.e"
+    write_exodus(path, P["coords"], [(exo + 1, "HEX8" if dim == 3 else "QUAD4")], elem_vars=ev)
+    d, rc, rxyz, _ = exodus.read_mesh(path)
+    _, centres, variables = exodus.read_model(path)
+    par = model.vertex_parameters(d, physics, rc, rxyz, centres, variables)
+    np.testing.assert_array_equal(par, P["par"])
+    M0 = sem.Mesh(dim, 4, physics, cells, P["coords"], P["par"])
+    M1 = sem.Mesh(d, 4, physics, rc, rxyz, par)
+    assert (M1.ntiles > 0) == (not distorted)
+    np.testing.assert_array_equal(run(M1, P), run(M0, P))
