@@ -521,24 +521,6 @@ int capture_graph(sem_batch *b) {
 
 }  // namespace sem
 
-// Fused-path stiffness-only timing of the current launch shape (no state
-// changes besides the accumulator).
-static double time_shape(sem_batch *b, int reps) {
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1;
-  if (enqueue_fused_step(b, 0, 1)) return -1;  // warm
-  (void)hipEventRecord(e0, b->stream);
-  for (int r = 0; r < reps; ++r)
-    if (enqueue_fused_step(b, 0, 1)) return -1;
-  (void)hipEventRecord(e1, b->stream);
-  (void)hipEventSynchronize(e1);
-  float ms = 0;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  return ms;
-}
-
 // Full batched-path steps (state advances; step counters do not), each timed
 // with HIP events after one warm step; returns the fastest step's
 // milliseconds (the least disturbed sample).
@@ -806,25 +788,32 @@ int sem_batch_prepare(sem_batch *b, int path, int epb, int spg) {
       nd_max = std::max(nd_max, g.mesh->nd);
     }
     if (any_elem) {
-      // autotune elements per block of the element kernel on the
-      // stiffness-only pass: one candidate per block size (the most elements
-      // that fit in bt threads)
+      // autotune elements per block of the element kernel over full steps
+      // (gather, operator and finish: the stiffness pass alone ranks them
+      // differently), one candidate per distinct block size; the timed steps
+      // advance the state, so it is rebuilt from the shot specifications
       double best = 1e300;
-      int best_epb = b->epb;
+      int best_epb = b->epb, last = 0;
       for (int bt = 64; bt <= 512; bt += 64) {
         const int cand = bt / nd_max;
-        if (cand < 1) continue;
+        if (cand < 1 || cand == last) continue;
+        last = cand;
         b->epb = cand;
         if ((rc = build_plan(b))) return fail(rc);
-        const double t = time_shape(b, 3);
+        const double t = time_steps(b, 3);
         if (t < 0) return fail(-2);
-        if (t < best * 0.98) {
+        if (t < best * 0.99) {
           best = t;
           best_epb = cand;
         }
       }
-      b->epb = best_epb;
-      if ((rc = build_plan(b))) return fail(rc);
+      batch_release(b);
+      b->path = path;
+      form_groups(b, max_s);
+      b->h_groups.resize(b->groups.size());
+      for (int gi = 0; gi < (int)b->groups.size(); ++gi)
+        if ((rc = setup_group(b, gi))) return fail(rc);
+      if ((rc = finalize_groups(b, best_epb))) return fail(rc);
     }
   }
   b->spg = max_s;

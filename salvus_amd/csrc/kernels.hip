@@ -280,7 +280,8 @@ struct NodeCtx {
 template <int DIM, int PHYS>
 struct NodeMat {
   // acoustic: G[6] (already times w vp^2);
-  // elastic : Jinv[DIM*DIM], wdet, c[9] (2D uses c[0..3] = C11,C13,C33,C55)
+  // elastic : Jinv[DIM*DIM], wdet, c[9] (2D uses c[0..3] = C11,C13,C33,C55),
+  //           the coefficients times wdet
   float G[6];
   float Jinv[9];
   float wdet;
@@ -352,6 +353,12 @@ __device__ __forceinline__ void load_mat(const NodeCtx<DIM, N> &X, int64_t e, gp
       for (int k = 0; k < (DIM == 2 ? 4 : 9); ++k) M.c[k] = gp[(DIM * DIM + 1 + k) * S::ND];
     }
   }
+  if (PHYS != SEM_ACOUSTIC) {
+    // the stress is linear in the coefficients: scaling them by w det J once
+    // per element visit leaves the flux without a scaling of its own
+#pragma unroll
+    for (int k = 0; k < (DIM == 2 ? 4 : 9); ++k) M.c[k] *= M.wdet;
+  }
 }
 
 // Stage 1: fluxes of this node into LDS.  su: C x ND field, sF: C x DIM x ND.
@@ -411,7 +418,7 @@ __device__ __forceinline__ void node_flux(const NodeCtx<DIM, N> &X, const NodeMa
         float s = 0.f;
 #pragma unroll
         for (int b = 0; b < DIM; ++b) s = fmaf(M.Jinv[b * DIM + a], sig[c][b], s);
-        sF[(c * DIM + a) * S::ND + X.p] = M.wdet * s;
+        sF[(c * DIM + a) * S::ND + X.p] = s;
       }
   }
 }
@@ -473,14 +480,16 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
   const bool first = (word & kFlagFirst) != 0;
   const bool fin = (word & kFlagLast) && !A.ku_only;
   for (int s = 0; s < Sg; ++s) {
-    // gather, and this node's finish inputs loaded ahead of the element work
+    // gather (shot 0 only: every later shot's u is loaded while the previous
+    // shot integrates and finishes, and staged after this thread's own
+    // finish), and this node's finish inputs loaded ahead of the element work
     float pacc[C], pap[C], pv[C];
     uint32_t pkc[C];
     if (active) {
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         const int64_t idx = ((int64_t)s * C + c) * G.G + g;
-        su[c * S::ND + p] = gu[idx];
+        if (s == 0) su[c * S::ND + p] = gu[idx];
         pacc[c] = first ? 0.f : gacc[idx];
         pap[c] = fin ? gap[idx] : 0.f;
         pv[c] = fin ? gv[idx] : 0.f;
@@ -491,6 +500,10 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
     if (active) node_flux<DIM, N, PHYS>(X, M, su, sF);
     __syncthreads();
     if (active) {
+      float nu[C];
+      const bool more = s + 1 < Sg;
+#pragma unroll
+      for (int c = 0; c < C; ++c) nu[c] = more ? gu[((int64_t)(s + 1) * C + c) * G.G + g] : 0.f;
       float f[C];
 #pragma unroll
       for (int c = 0; c < C; ++c) {
@@ -531,6 +544,12 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
           }
           gu[idx] = u; gv[idx] = v; gap[idx] = a; gkc[idx] = pack_comp(ku, kv);
         }
+      }
+      // own slot only: every thread's flux stage has read su (barrier above)
+      // and this thread's finish has read its u
+      if (more) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) su[c * S::ND + p] = nu[c];
       }
     }
     __syncthreads();
