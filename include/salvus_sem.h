@@ -145,7 +145,10 @@ int sem_shot_add_receiver(sem_batch *batch, int shot, const double *loc);
 int sem_shot_set_field(sem_batch *batch, int shot, int which, const double *vals);
 int sem_shot_get_field(sem_batch *batch, int shot, int which, double *vals);
 /* Allocates device state and records the step plan.  path: SEM_PATH_*;
- * elems_per_block / shots_per_group: 0 = autotuned. */
+ * elems_per_block / shots_per_group: 0 = autotuned (each candidate timed
+ * over full steps; the timed steps' state is discarded and rebuilt from the
+ * shot specifications, which include initial fields set before this call).
+ * No reference counterpart (the reference runs one problem per process). */
 int sem_batch_prepare(sem_batch *batch, int path, int elems_per_block, int shots_per_group);
 /* Advances every unfinished shot by up to nsteps steps. */
 int sem_batch_run(sem_batch *batch, int64_t nsteps);
