@@ -30,6 +30,7 @@
 // so results are reproducible bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <cstring>
 
 #include "sem_internal.h"
@@ -776,6 +777,7 @@ int tile_kernel_prepare(int dim, int order, int physics, const int *T, int *per_
 // pays only where the extra resident workgroup saves a round (1.4x faster
 // at 703 tiles per colour on 256 CUs, 4 % slower at 312).  Both variants
 // do the same arithmetic in the same order: results are identical.
+// SEM_TILE_OCCUPANCY=normal|high forces one variant (tests compare them).
 template <int D, int N, int P, int X, int Y, int Z>
 static int go_tile(int grid, const FusedArgs &a, hipStream_t st) {
   using S = TS<D, N, P, X, Y, Z, false>;
@@ -785,7 +787,11 @@ static int go_tile(int grid, const FusedArgs &a, hipStream_t st) {
     if (!occ) { set_error("tile occupancy query failed"); return -2; }
     const int64_t lo = ((int64_t)grid + (int64_t)occ[0] * occ[1] - 1) / ((int64_t)occ[0] * occ[1]);
     const int64_t hi = ((int64_t)grid + (int64_t)occ[0] * occ[2] - 1) / ((int64_t)occ[0] * occ[2]);
-    if (6 * hi < 5 * lo) {  // 1.2 hi < lo
+    const char *force = getenv("SEM_TILE_OCCUPANCY");
+    const bool high = force && !strcmp(force, "high")     ? true
+                      : force && !strcmp(force, "normal") ? false
+                                                          : 6 * hi < 5 * lo;  // 1.2 hi < lo
+    if (high) {
       hipLaunchKernelGGL((k_tile<D, N, P, X, Y, Z, true>), dim3(grid), dim3(S::BT), shm, st, a);
       return 0;
     }
