@@ -25,13 +25,14 @@ I32 = C.POINTER(C.c_int32)
 I64 = C.POINTER(C.c_int64)
 
 
-def build():
+def build(force=False):
     """Compile liboracle.so (and the reference GLL table library when the
-    reference sources are present)."""
+    reference sources are present); force recompiles everything."""
     import subprocess
-    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    flags = ["-B"] if force else []
+    subprocess.run(["make", "-s"] + flags + ["-C", str(HERE)], check=True)
     if Path("/root/reference/src").is_dir():
-        subprocess.run(["make", "-s", "-C", str(HERE), "ref"], check=True)
+        subprocess.run(["make", "-s"] + flags + ["-C", str(HERE), "ref"], check=True)
 
 
 def lib():
