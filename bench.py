@@ -254,7 +254,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded models and sources)",
         "config": {"workload": desc, "meshes": [int(M.E) for M in meshes], "shots": len(ids),
-                   "dof_updates_per_step": dofs, "path": "batched (tile kernel)",
+                   "dof_updates_per_step": dofs,
+                   "path": "batched (%s)" % ("tile kernel" if all(M.tile_shape for M in meshes)
+                                             else "element kernel" if not any(M.tile_shape for M in meshes)
+                                             else "tile and element kernels"),
                    "tile_shapes": [list(M.tile_shape) if M.tile_shape else None for M in meshes],
                    "shots_per_group": cfg["shots_per_group"], "groups": cfg["groups"],
                    "graph_block_steps": cfg["block_steps"]},
