@@ -36,12 +36,20 @@ typedef struct sem_server sem_server;
 
 enum { SEM_ACOUSTIC = 0, SEM_ELASTIC = 1 };
 
-/* Reference-behaviour switches: SEM_QUIRK_E3D_SOURCE keeps Elastic3D's source
- * scaling (its source term skips test-and-integrate, Elastic3D.cpp:126-133);
- * SEM_QUIRK_NEWMARK3D advances Elastic3D's x twice and z never, as the field
- * list {ax, ay, ax, a} of Order2Newmark.cpp:93-96 does. */
-enum { SEM_QUIRK_NEWMARK3D = 1, SEM_QUIRK_E3D_SOURCE = 2 };
-#define SEM_DEFAULT_QUIRKS SEM_QUIRK_E3D_SOURCE
+/* Reference-behaviour switches:
+ * SEM_QUIRK_E3D_SOURCE keeps Elastic3D's source scaling (its source term
+ *   skips test-and-integrate, Elastic3D.cpp:126-133);
+ * SEM_QUIRK_NEWMARK3D follows the field list {ax, ay, ax, a} of
+ *   Order2Newmark.cpp:93-96: on elastic meshes (2-D and 3-D) x advances twice
+ *   per step, y once and z never (the name keeps its first use);
+ * SEM_QUIRK_QUAD_JACOBIAN gives non-parallelogram quads the reference's
+ *   operator: the Jacobian of QuadP1.h:20-27 (row d/dr from dN/dr evaluated
+ *   at r, row d/ds from dN/ds at s), J^-1 applied at the output node
+ *   (TensorQuad.cpp:383-390) and that det J in the mass.  Parallelograms and
+ *   hexes are unaffected (both Jacobians are one there).  Off: the consistent
+ *   bilinear Jacobian with J^-T at the quadrature points. */
+enum { SEM_QUIRK_NEWMARK3D = 1, SEM_QUIRK_E3D_SOURCE = 2, SEM_QUIRK_QUAD_JACOBIAN = 4 };
+#define SEM_DEFAULT_QUIRKS (SEM_QUIRK_E3D_SOURCE | SEM_QUIRK_QUAD_JACOBIAN)
 
 /* Execution paths of a batch. */
 enum {

@@ -29,7 +29,10 @@ their shared nodes:
   - the edge integral (TensorQuad.cpp:474-515) weights the edge nodes by
     w_j |edge| / 2;
 * the velocities in s are those after the previous step;
-* Newmark as Order2Newmark.cpp:68-115 on every field.
+* Newmark as Order2Newmark.cpp:68-115 on every field; with
+  `newmark_quirk` the reference's literal field list {ax, ay, ax, a}
+  (Order2Newmark.cpp:93-96): the solid's x advanced twice, y once, the
+  fluid's a once.
 """
 import numpy as np
 
@@ -42,7 +45,8 @@ def _edge_nodes(i, n):
 
 
 class CoupledSim:
-    def __init__(self, order, cells, coords, fluid, par_fluid, rho_fluid, par_solid):
+    def __init__(self, order, cells, coords, fluid, par_fluid, rho_fluid, par_solid,
+                 newmark_quirk=False):
         """cells E x 4 (reference vertex order), coords V x 2, fluid[E] bool;
         par_fluid / rho_fluid: (fluid elements) x 4 x 1 VP and x 4 RHO;
         par_solid: (solid elements) x 4 x 5 (RHO, C11, C13, C33, C55)."""
@@ -50,6 +54,7 @@ class CoupledSim:
         coords = np.asarray(coords, dtype=np.float64)
         fluid = np.asarray(fluid, dtype=bool)
         self.order, self.n = order, order + 1
+        self.newmark_quirk = bool(newmark_quirk)
         self.cells, self.coords, self.fluid = cells, coords, fluid
         self.ef = np.flatnonzero(fluid)
         self.es = np.flatnonzero(~fluid)
@@ -155,8 +160,11 @@ class CoupledSim:
                 acc[key] = a * P["mi"][None, :]
             for key, P in self.part.items():
                 a = acc[key]
-                P["v"] += acl * a + acl * P["ap"]
-                P["u"] += dt * P["v"] + dsp * a
-                P["ap"] = a
+                # components in the order the reference's field list visits them
+                comps = [0, 1, 0] if (self.newmark_quirk and P["c"] >= 2) else range(P["c"])
+                for k in comps:
+                    P["v"][k] += acl * a[k] + acl * P["ap"][k]
+                    P["u"][k] += dt * P["v"][k] + dsp * a[k]
+                    P["ap"][k] = a[k]
             t += dt
         return {k: P["u"].copy() for k, P in self.part.items()}

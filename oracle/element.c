@@ -20,12 +20,20 @@ int orc_tables(orc_tab *T, int dim, int order) {
   if (order < 1 || order > (dim == 2 ? 10 : 7)) return -1;
   T->dim = dim; T->order = order; T->n = order + 1;
   T->nd = dim == 2 ? T->n * T->n : T->n * T->n * T->n;
+  T->quad_ref = 1;
   orc_gll(order, T->xi, T->w);
   orc_deriv_matrix(order, T->D);
   return 0;
 }
 
 /* ---------------- quads ---------------- */
+
+/* The quad Jacobian the element's mode takes (see orc_tab.quad_ref). */
+static int quad_jac(const orc_tab *T, double r, double s, const double *vtx, double *det,
+                    double *inv) {
+  return T->quad_ref ? orc_quad_invjac(r, s, vtx, det, inv)
+                     : orc_quad_invjac_consistent(r, s, vtx, det, inv);
+}
 
 /* TensorQuad.cpp:176-210; writes detJ per point (TensorQuad.cpp:194: the
  * gradient leaves mDetJac for applyGradTestAndIntegrate, :371-374). */
@@ -36,7 +44,7 @@ static void quad_gradient(const orc_tab *T, const double *vtx, const double *u, 
     for (int r = 0; r < n; r++) {
       int idx = r + s * n;
       double inv[4];
-      orc_quad_invjac(T->xi[r], T->xi[s], vtx, &detJ[idx], inv);
+      quad_jac(T, T->xi[r], T->xi[s], vtx, &detJ[idx], inv);
       double g0 = 0, g1 = 0;
       for (int i = 0; i < n; i++) {
         g0 += T->D[r * n + i] * u[i + s * n];
@@ -47,11 +55,35 @@ static void quad_gradient(const orc_tab *T, const double *vtx, const double *u, 
     }
 }
 
-/* TensorQuad.cpp:350-397; f is n^2 x 2 row-major. */
+/* TensorQuad.cpp:350-397; f is n^2 x 2 row-major.  In the consistent mode
+ * the flux takes J^-T at each quadrature point first and the output node
+ * takes no transform (the hexes' order, Hexahedra.cpp:688-754). */
 static void quad_grad_test_integrate(const orc_tab *T, const double *vtx, const double *detJ,
                                      const double *f, double *out) {
   int n = T->n;
   const double *w = T->w, *D = T->D;
+  if (!T->quad_ref) {
+    double fr[2 * ORC_NDMAX];
+    for (int s = 0; s < n; s++)
+      for (int r = 0; r < n; r++) {
+        int idx = r + s * n;
+        double det, inv[4];
+        orc_quad_invjac_consistent(T->xi[r], T->xi[s], vtx, &det, inv);
+        fr[2 * idx + 0] = inv[0] * f[2 * idx] + inv[2] * f[2 * idx + 1];
+        fr[2 * idx + 1] = inv[1] * f[2 * idx] + inv[3] * f[2 * idx + 1];
+      }
+    for (int s = 0; s < n; s++)
+      for (int r = 0; r < n; r++) {
+        double d0 = 0, d1 = 0;
+        for (int i = 0; i < n; i++) {
+          int ri = i + s * n, si = r + i * n;
+          d0 += detJ[ri] * fr[2 * ri + 0] * D[i * n + r] * w[i];
+          d1 += detJ[si] * fr[2 * si + 1] * D[i * n + s] * w[i];
+        }
+        out[r + s * n] = d0 * w[s] + d1 * w[r];
+      }
+    return;
+  }
   for (int s = 0; s < n; s++)
     for (int r = 0; r < n; r++) {
       double fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0;
@@ -64,7 +96,7 @@ static void quad_grad_test_integrate(const orc_tab *T, const double *vtx, const 
       }
       fx0 *= w[s]; fx1 *= w[r]; fy0 *= w[s]; fy1 *= w[r];
       double det, inv[4];
-      orc_quad_invjac(T->xi[r], T->xi[s], vtx, &det, inv);
+      quad_jac(T, T->xi[r], T->xi[s], vtx, &det, inv);
       out[r + s * n] = (inv[0] * fx0 + inv[1] * fx1) + (inv[2] * fy0 + inv[3] * fy1);
     }
 }
@@ -159,7 +191,7 @@ void orc_test_integrate(const orc_tab *T, const double *vtx, const double *f, do
     for (int s = 0; s < n; s++)
       for (int r = 0; r < n; r++) {
         double det;
-        orc_quad_invjac(T->xi[r], T->xi[s], vtx, &det, 0);
+        quad_jac(T, T->xi[r], T->xi[s], vtx, &det, 0);
         int idx = r + s * n;
         out[idx] = f[idx] * det * T->w[r] * T->w[s];
       }
@@ -195,14 +227,16 @@ void orc_delta_coeffs(const orc_tab *T, const double *vtx, const double *ref, do
   for (int p = 0; p < T->nd; p++) {
     int i = p % n, j = (p / n) % n, k = p / (n * n);
     double det;
-    if (T->dim == 2) orc_quad_invjac(T->xi[i], T->xi[j], vtx, &det, 0);
+    if (T->dim == 2) quad_jac(T, T->xi[i], T->xi[j], vtx, &det, 0);
     else orc_hex_invjac(T->xi[i], T->xi[j], T->xi[k], vtx, &det, 0);
     out[p] /= (T->w[i] * T->w[j] * (T->dim == 3 ? T->w[k] : 1.0) * det);
   }
 }
 
 /* Physical gradient of every basis function at a reference point:
- * dphi[p*d + a] = d phi_p / d x_a (J^-1 grad_r, QuadP1.h:79-90 / HexP1.cpp:189). */
+ * dphi[p*d + a] = d phi_p / d x_a (J^-1 grad_r, HexP1.cpp:189).  Used by the
+ * moment-tensor source, which has no reference counterpart: quads take the
+ * consistent bilinear Jacobian in either mode. */
 void orc_basis_gradient_at(const orc_tab *T, const double *vtx, const double *ref,
                            double *dphi) {
   int n = T->n, d = T->dim;
@@ -211,7 +245,7 @@ void orc_basis_gradient_at(const orc_tab *T, const double *vtx, const double *re
     orc_lagrange(n, T->xi, ref[a], l[a]);
     orc_lagrange_deriv(n, T->xi, ref[a], dl[a]);
   }
-  if (d == 2) orc_quad_invjac(ref[0], ref[1], vtx, &det, inv);
+  if (d == 2) orc_quad_invjac_consistent(ref[0], ref[1], vtx, &det, inv);
   else orc_hex_invjac(ref[0], ref[1], ref[2], vtx, &det, inv);
   for (int p = 0; p < T->nd; p++) {
     int i = p % n, j = (p / n) % n, k = p / (n * n);

@@ -29,12 +29,21 @@ enum { ORC_ACOUSTIC = 0, ORC_ELASTIC = 1 };
 
 /* Reference-behaviour switches (bitmask).
  * ORC_QUIRK_NEWMARK3D  : Order2Newmark.cpp:93-96 iterates {ax,ay,ax,a}; for
- *                        Elastic3D the x component is advanced twice per step
- *                        and z never.  Off = every component advanced once.
+ *                        Elastic2D and Elastic3D the x component is advanced
+ *                        twice per step, y once and z never.  Off = every
+ *                        component advanced once.
  * ORC_QUIRK_E3D_SOURCE : Elastic3D.cpp:126-133 skips applyTestAndIntegrate,
  *                        so its nodal load is l_i(xs)/(w_i detJ_i) f(t).
- *                        Off = l_i(xs) f(t) as Scalar/Elastic2D give. */
-enum { ORC_QUIRK_NEWMARK3D = 1, ORC_QUIRK_E3D_SOURCE = 2 };
+ *                        Off = l_i(xs) f(t) as Scalar/Elastic2D give.
+ * ORC_QUIRK_QUAD_JACOBIAN : quads take the reference's operator: the
+ *                        Jacobian of QuadP1.h:20-27 (row d/dr built from
+ *                        dN/dr evaluated at r, row d/ds from dN/ds at s;
+ *                        exact for parallelograms only), J^-1 applied at the
+ *                        output node (TensorQuad.cpp:383-390) and that det J
+ *                        in mass and source integration.  Off = consistent
+ *                        bilinear Jacobian, J^-T at the quadrature points.
+ */
+enum { ORC_QUIRK_NEWMARK3D = 1, ORC_QUIRK_E3D_SOURCE = 2, ORC_QUIRK_QUAD_JACOBIAN = 4 };
 
 /* ---------------- GLL quadrature (a19) ---------------- */
 /* Reference literals (Quad/Autogen/order<N>_square.c); 0 on success. */
@@ -49,6 +58,9 @@ int orc_deriv_matrix(int order, double *D);
 
 /* ---------------- P1 geometry (a9, a10, a21, a24) ---------------- */
 int orc_quad_invjac(double r, double s, const double *vtx, double *det, double *inv);
+/* Consistent bilinear Jacobian of the P1 map (QuadP1.h:98-108 shape
+ * functions differentiated): row d/dr uses dN/dr(s), row d/ds dN/ds(r). */
+int orc_quad_invjac_consistent(double r, double s, const double *vtx, double *det, double *inv);
 void orc_quad_interp(double r, double s, double *w4);
 int orc_quad_check_hull(double x, double y, const double *vtx);
 int orc_quad_inverse_map(double x, double y, const double *vtx, double *rs);

@@ -7,6 +7,11 @@
 
 typedef struct {
   int dim, order, n, nd;
+  /* quads: 1 = the reference's operator (QuadP1.h:20-27 Jacobian rows with
+   * each derivative evaluated at the other coordinate; J^-1 applied at the
+   * output node, TensorQuad.cpp:383-390), 0 = the consistent bilinear
+   * Jacobian with J^-T at the quadrature points (as the hexes do) */
+  int quad_ref;
   double xi[16], w[16], D[16 * 16];
 } orc_tab;
 

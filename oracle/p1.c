@@ -41,6 +41,27 @@ int orc_quad_invjac(double r, double s, const double *vtx, double *det, double *
   return d > 0 ? 0 : -1;
 }
 
+int orc_quad_invjac_consistent(double r, double s, const double *vtx, double *det, double *inv) {
+  const double dr[4] = {-(1 - s) / 4.0, (1 - s) / 4.0, (1 + s) / 4.0, -(1 + s) / 4.0};
+  const double ds[4] = {-(1 - r) / 4.0, -(1 + r) / 4.0, (1 + r) / 4.0, (1 - r) / 4.0};
+  double J[2][2] = {{0, 0}, {0, 0}};
+  for (int v = 0; v < 4; v++) {
+    J[0][0] += dr[v] * vtx[2 * v + 0];
+    J[0][1] += dr[v] * vtx[2 * v + 1];
+    J[1][0] += ds[v] * vtx[2 * v + 0];
+    J[1][1] += ds[v] * vtx[2 * v + 1];
+  }
+  double d = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  *det = d;
+  if (inv) {
+    inv[0] = J[1][1] / d;
+    inv[1] = -J[0][1] / d;
+    inv[2] = -J[1][0] / d;
+    inv[3] = J[0][0] / d;
+  }
+  return d > 0 ? 0 : -1;
+}
+
 /* QuadP1.h:98-108 */
 void orc_quad_interp(double r, double s, double *w4) {
   w4[0] = +0.25 * r * s - 0.25 * r - 0.25 * s + 0.25;

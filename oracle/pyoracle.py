@@ -15,8 +15,8 @@ LIB_PATH = HERE / "liboracle.so"
 REF_LIB_PATH = HERE / "_ref" / "libgll_ref.so"
 
 ACOUSTIC, ELASTIC = 0, 1
-QUIRK_NEWMARK3D, QUIRK_E3D_SOURCE = 1, 2
-DEFAULT_QUIRKS = QUIRK_E3D_SOURCE
+QUIRK_NEWMARK3D, QUIRK_E3D_SOURCE, QUIRK_QUAD_JACOBIAN = 1, 2, 4
+DEFAULT_QUIRKS = QUIRK_E3D_SOURCE | QUIRK_QUAD_JACOBIAN
 
 _lib = None
 
@@ -87,6 +87,7 @@ def lib():
         L.orc_lagrange.argtypes = [C.c_int, D, C.c_double, D]
         L.orc_lagrange_deriv.argtypes = [C.c_int, D, C.c_double, D]
         L.orc_quad_invjac.argtypes = [C.c_double, C.c_double, D, D, D]
+        L.orc_quad_invjac_consistent.argtypes = [C.c_double, C.c_double, D, D, D]
         L.orc_quad_interp.argtypes = [C.c_double, C.c_double, D]
         L.orc_quad_check_hull.argtypes = [C.c_double, C.c_double, D]
         L.orc_quad_inverse_map.argtypes = [C.c_double, C.c_double, D, D]
@@ -164,6 +165,13 @@ def quad_invjac(r, s, vtx):
     v = f64(vtx).reshape(-1)
     det, inv = np.zeros(1), np.zeros(4)
     lib().orc_quad_invjac(r, s, dp(v), dp(det), dp(inv))
+    return det[0], inv.reshape(2, 2)
+
+
+def quad_invjac_consistent(r, s, vtx):
+    v = f64(vtx).reshape(-1)
+    det, inv = np.zeros(1), np.zeros(4)
+    lib().orc_quad_invjac_consistent(r, s, dp(v), dp(det), dp(inv))
     return det[0], inv.reshape(2, 2)
 
 

@@ -82,6 +82,7 @@ orc_sim *orc_sim_new(int dim, int order, int physics, int64_t E, int64_t V,
   if (orc_tables(&S->T, dim, order)) { set_err("unsupported dim/order"); free(S); return NULL; }
   if (physics != ORC_ACOUSTIC && physics != ORC_ELASTIC) { set_err("bad physics"); free(S); return NULL; }
   S->physics = physics; S->quirks = quirks;
+  S->T.quad_ref = (quirks & ORC_QUIRK_QUAD_JACOBIAN) ? 1 : 0;
   S->c = physics == ORC_ACOUSTIC ? 1 : dim;
   S->nv = dim == 2 ? 4 : 8;
   S->npar = npar_of(dim, physics);
@@ -400,10 +401,17 @@ int orc_sim_run(orc_sim *S, double dt, int64_t nsteps, double *seis, int nthread
     for (int comp = 0; comp < c; comp++)
       for (int64_t g = 0; g < G; g++) S->a[comp * G + g] = S->mi[g] * S->a[comp * G + g];
     /* Order2Newmark.cpp:85-115 */
-    int order3q[4] = {0, 1, 0, -1};
+    /* With the quirk, the reference's literal field list {ax, ay, ax, a}
+     * (Order2Newmark.cpp:93-96), each entry taken when the field exists:
+     * elastic meshes (2-D and 3-D) advance x twice, y once and z never;
+     * acoustic meshes advance "a" once.  Without it every component once. */
     int ncomp_list = c;
     int list[4] = {0, 1, 2, -1};
-    if (c == 3 && (S->quirks & ORC_QUIRK_NEWMARK3D)) { memcpy(list, order3q, sizeof list); ncomp_list = 3; }
+    if (c >= 2 && (S->quirks & ORC_QUIRK_NEWMARK3D)) {
+      const int lit[3] = {0, 1, 0};
+      memcpy(list, lit, sizeof lit);
+      ncomp_list = 3;
+    }
     for (int li = 0; li < ncomp_list; li++) {
       int comp = list[li];
       double *u = S->u + comp * G, *v = S->v + comp * G, *a = S->a + comp * G, *ap = S->a_ + comp * G;

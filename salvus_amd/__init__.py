@@ -4,5 +4,5 @@ Element/Physics/Problem interface, through the C ABI of include/salvus_sem.h.
 """
 from . import meshgen  # noqa: F401
 from .sem import (ACOUSTIC, ELASTIC, DEFAULT_QUIRKS, PATH_FUSED, PATH_PLAIN,  # noqa: F401
-                  QUIRK_E3D_SOURCE, QUIRK_NEWMARK3D, Batch, Mesh, Order2Newmark, SemError,
+                  QUIRK_E3D_SOURCE, QUIRK_NEWMARK3D, QUIRK_QUAD_JACOBIAN, Batch, Mesh, Order2Newmark, SemError,
                   device_count, reference_step_count, topology)

@@ -118,9 +118,10 @@ static int upload_conn(sem_mesh *m) {
 
 int sem::ensure_pointwise(sem_mesh *m) {
   if (m->d_geo_p) return 0;
-  const int ngp = point_geo_count(m->dim, m->physics);
+  const int gm = pointwise_mode(m);
+  const int ngp = point_geo_count(m->dim, m->physics, gm);
   SEM_TRY(dalloc(&m->d_geo_p, (size_t)m->E * ngp * m->nd, &m->bytes));
-  SEM_TRY(launch_point_geometry(m->dim, m->order, m->physics, m->E, m->d_vtx, m->d_vpar,
+  SEM_TRY(launch_point_geometry(m->dim, m->order, m->physics, gm, m->E, m->d_vtx, m->d_vpar,
                                 ref_npar(m->dim, m->physics), m->d_geo_p, nullptr, m->stream));
   HIPCHK(hipStreamSynchronize(m->stream));
   return 0;
@@ -424,7 +425,8 @@ int sem_mesh_create(sem_mesh **out, int device, int dim, int order, int physics,
     double *me = nullptr, *M = nullptr;
     if ((rc = dalloc(&me, (size_t)E * m->nd))) return fail(rc);
     if ((rc = dalloc(&M, (size_t)m->G))) { dfree(me); return fail(rc); }
-    rc = launch_point_geometry(dim, order, physics, E, m->d_vtx, m->d_vpar, npar, nullptr, me, m->stream);
+    rc = launch_point_geometry(dim, order, physics, pointwise_mode(m), E, m->d_vtx, m->d_vpar, npar,
+                               nullptr, me, m->stream);
     if (!rc) rc = launch_fill_f64(m->G, M, 0.0, m->stream);
     for (int c = 0; c < m->ncol && !rc; ++c)
       rc = launch_mass_scatter(m->col_start[c], m->col_start[c + 1] - m->col_start[c], m->nd,

@@ -255,13 +255,13 @@ static void batch_release(sem_batch *b) {
 // Work items per (kernel key, colour): tiles of tiled groups, EPB-element
 // slices of the element kernel's colour ranges otherwise.
 static int build_plan(sem_batch *b) {
-  // key = (tile?, dim, order, physics, affine, tile shape)
+  // key = (tile?, dim, order, physics, geometry mode, tile shape)
   std::map<std::tuple<int, int, int, int, int, int>, std::vector<int>> keys;
   for (int gi = 0; gi < (int)b->groups.size(); ++gi) {
     sem_mesh *m = b->groups[gi].mesh;
     const bool t = b->groups[gi].tiled;
     const int shape = t ? m->tiles.T[0] * 10000 + m->tiles.T[1] * 100 + m->tiles.T[2] : 0;
-    keys[std::make_tuple((int)t, m->dim, m->order, m->physics, (int)m->affine, shape)].push_back(gi);
+    keys[std::make_tuple((int)t, m->dim, m->order, m->physics, geo_mode(m), shape)].push_back(gi);
   }
   for (auto &l : b->launches) dfree(l.d_items);
   b->launches.clear();
@@ -307,7 +307,7 @@ static int build_plan(sem_batch *b) {
       L.dim = std::get<1>(kv.first);
       L.order = std::get<2>(kv.first);
       L.physics = std::get<3>(kv.first);
-      L.affine = std::get<4>(kv.first) != 0;
+      L.gmode = std::get<4>(kv.first);
       L.nd = L.dim == 2 ? (L.order + 1) * (L.order + 1) : (L.order + 1) * (L.order + 1) * (L.order + 1);
       L.c = L.physics == SEM_ACOUSTIC ? 1 : L.dim;
       L.colour = c;
@@ -447,7 +447,7 @@ int enqueue_fused_step(sem_batch *b, int koff, int ku_only) {
     a.ku_only = ku_only;
     auto go = [&]() {
       return L.tile ? launch_tile(L.dim, L.order, L.physics, L.T, L.grid, a, b->stream)
-                    : launch_fused(L.dim, L.order, L.physics, L.affine, L.nitems, b->epb, L.nd, L.c, a, b->stream);
+                    : launch_fused(L.dim, L.order, L.physics, L.gmode, L.nitems, b->epb, L.nd, L.c, a, b->stream);
     };
     SEM_TRY(go());
   }
@@ -484,10 +484,10 @@ int enqueue_plain_step(sem_batch *b) {
       b->ev.push_back(e1);
       ++b->ev_launches;
       HIPCHK(hipEventRecord(e0, b->stream));
-      SEM_TRY(launch_plain_stiffness(m->dim, m->order, m->physics, m->E, m->nd, m->c, a, b->stream));
+      SEM_TRY(launch_plain_stiffness(m->dim, m->order, m->physics, pointwise_mode(m), m->E, m->nd, m->c, a, b->stream));
       HIPCHK(hipEventRecord(e1, b->stream));
     } else {
-      SEM_TRY(launch_plain_stiffness(m->dim, m->order, m->physics, m->E, m->nd, m->c, a, b->stream));
+      SEM_TRY(launch_plain_stiffness(m->dim, m->order, m->physics, pointwise_mode(m), m->E, m->nd, m->c, a, b->stream));
     }
     SEM_TRY(launch_fill_f32((int64_t)m->c * m->G, h.acc, 0.f, b->stream));
     for (int c = 0; c < m->ncol; ++c)
@@ -967,7 +967,7 @@ int sem_mesh_apply_stiffness(sem_mesh *m, int path, const float *u, float *ku) {
       a.aloc = b->d_aloc;
       a.epb = default_epb(m->nd);
       a.E = m->E;
-      if (!rc) rc = launch_plain_stiffness(m->dim, m->order, m->physics, m->E, m->nd, m->c, a, b->stream);
+      if (!rc) rc = launch_plain_stiffness(m->dim, m->order, m->physics, pointwise_mode(m), m->E, m->nd, m->c, a, b->stream);
       if (!rc) rc = launch_fill_f32((int64_t)len, h.acc, 0.f, b->stream);
       for (int c = 0; c < m->ncol && !rc; ++c)
         rc = launch_plain_scatter(m->col_start[c], m->col_start[c + 1] - m->col_start[c], m->nd, m->c,

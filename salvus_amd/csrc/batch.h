@@ -76,7 +76,7 @@ struct Group {
 
 struct LaunchSet {
   int dim, order, physics, nd, c, colour;
-  bool affine;
+  int gmode;  // geometry mode of the element kernel (kGeo*)
   bool tile;
   int nitems;
   int T[3];  // tile shape (tile kernel)

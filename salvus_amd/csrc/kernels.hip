@@ -51,7 +51,11 @@ __device__ __forceinline__ float p1_weight(int dim, int v, const float *xi3) {
 // Affine per-element factors from the Jacobian at the element centre.
 // Acoustic: Gsym = detJ Jinv^T Jinv (upper triangle, row-major: xx xy [xz] yy [yz] [zz]).
 // Elastic : Jinv (dim^2, [a][b] = d r_b / d x_a ... stored as Jinv[a*dim+b]) and detJ.
-__device__ void dev_jacobian(int dim, const double *vtx, const double *ref, double *J) {
+// quad_ref: the rows of QuadP1.h:20-27, where dN/dr is evaluated at r and
+// dN/ds at s (each derivative of the bilinear map taken at the other
+// coordinate; equal to the consistent Jacobian on parallelograms).
+__device__ void dev_jacobian(int dim, const double *vtx, const double *ref, double *J,
+                             bool quad_ref = false) {
   const int qs[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
   const int hs[8][3] = {{-1, -1, -1}, {-1, 1, -1}, {1, 1, -1}, {1, -1, -1},
                         {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
@@ -60,8 +64,8 @@ __device__ void dev_jacobian(int dim, const double *vtx, const double *ref, doub
   for (int v = 0; v < nv; ++v) {
     double d[3];
     if (dim == 2) {
-      d[0] = 0.25 * qs[v][0] * (1 + ref[1] * qs[v][1]);
-      d[1] = 0.25 * qs[v][1] * (1 + ref[0] * qs[v][0]);
+      d[0] = 0.25 * qs[v][0] * (1 + ref[quad_ref ? 0 : 1] * qs[v][1]);
+      d[1] = 0.25 * qs[v][1] * (1 + ref[quad_ref ? 1 : 0] * qs[v][0]);
     } else {
       d[0] = 0.125 * hs[v][0] * (1 + ref[1] * hs[v][1]) * (1 + ref[2] * hs[v][2]);
       d[1] = 0.125 * hs[v][1] * (1 + ref[0] * hs[v][0]) * (1 + ref[2] * hs[v][2]);
@@ -131,7 +135,7 @@ __global__ void k_affine_geometry(int dim, int physics, int64_t E, const double 
 // Pointwise factors (general geometry) and FP64 element mass, one thread
 // per (element, node).  geo_p layout: [E][ngp][nd].  me: [E][nd].
 template <int DIM, int N>
-__global__ void k_point_geometry(int physics, int64_t E, const double *__restrict__ vtx,
+__global__ void k_point_geometry(int physics, int gmode, int64_t E, const double *__restrict__ vtx,
                                  const double *__restrict__ vpar, int npar,
                                  float *__restrict__ geo_p, double *__restrict__ me) {
   using S = Sh<DIM, N>;
@@ -146,15 +150,21 @@ __global__ void k_point_geometry(int physics, int64_t E, const double *__restric
   double J[9], inv[9];
   const double *ev = vtx + e * S::NV * DIM;
   const double *ep = vpar + e * S::NV * npar;
-  dev_jacobian(DIM, ev, ref, J);
+  const bool qref = DIM == 2 && gmode == kGeoQuadRef;
+  dev_jacobian(DIM, ev, ref, J, qref);
   double det = dev_invert(DIM, J, inv);
   // mass: Scalar.cpp:34-39 (no rho), Elastic*.cpp: rho at the node
   double rho = physics == SEM_ACOUSTIC ? 1.0 : dev_p1(DIM, ep, npar, 0, ref);
   if (me) me[e * S::ND + p] = wq * det * rho;
   if (!geo_p) return;
-  const int ngp = point_geo_count(DIM, physics);
+  const int ngp = point_geo_count(DIM, physics, qref ? kGeoQuadRef : kGeoPoint);
   float *g = geo_p + e * ngp * S::ND + p;
-  if (physics == SEM_ACOUSTIC) {
+  if (physics == SEM_ACOUSTIC && qref) {
+    // reference quad operator: J^-1 at the node and w det J vp^2
+    const double vp = dev_p1(DIM, ep, npar, 0, ref);
+    for (int i = 0; i < 4; ++i) g[i * S::ND] = (float)inv[i];
+    g[4 * S::ND] = (float)(wq * det * vp * vp);
+  } else if (physics == SEM_ACOUSTIC) {
     double vp = dev_p1(DIM, ep, npar, 0, ref);
     int k = 0;
     for (int a = 0; a < DIM; ++a)
@@ -264,6 +274,24 @@ struct NodeCtx {
       g[a] = s;
     }
   }
+  // test-and-integrate with J^-1 applied at this (output) node, as
+  // TensorQuad.cpp:350-397 does: F[a*ND + q] holds the physical-axis-a flux
+  // times w det J (no J^-T), and the result is
+  // sum_a sum_b Jinv[a][b] sum_m D[m][P_b] F_a(line b, m).
+  __device__ __forceinline__ float div_ref(const float *F, const float *Ji) const {
+    const int stride[3] = {1, S::n, S::n * S::n};
+    float s = 0.f;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+#pragma unroll
+      for (int b = 0; b < DIM; ++b) {
+        float d = 0.f;
+#pragma unroll
+        for (int m = 0; m < S::n; ++m) d = fmaf(Dcol[b][m], F[a * S::ND + base[b] + m * stride[b]], d);
+        s = fmaf(Ji[a * DIM + b], d, s);
+      }
+    return s;
+  }
   // test-and-integrate of LDS fluxes F[a*ND + q]
   __device__ __forceinline__ float div(const float *F) const {
     const int stride[3] = {1, S::n, S::n * S::n};
@@ -288,11 +316,12 @@ struct NodeMat {
   float c[9];
 };
 
-template <int DIM, int N, int PHYS, bool AFFINE>
+template <int DIM, int N, int PHYS, int GM>
 __device__ __forceinline__ void load_mat(const NodeCtx<DIM, N> &X, int64_t e, gptr<const float> geo_e,
                                          gptr<const float> par_e, gptr<const float> geo_p,
                                          NodeMat<DIM, PHYS> &M) {
   using S = Sh<DIM, N>;
+  constexpr bool AFFINE = GM == kGeoAffine;
   if (AFFINE) {
     float w8[8];
 #pragma unroll
@@ -340,9 +369,14 @@ __device__ __forceinline__ void load_mat(const NodeCtx<DIM, N> &X, int64_t e, gp
       }
     }
   } else {
-    constexpr int ngp = PHYS == SEM_ACOUSTIC ? DIM * (DIM + 1) / 2 : (DIM == 2 ? 9 : 19);
+    constexpr int ngp = PHYS == SEM_ACOUSTIC ? (GM == kGeoQuadRef ? 5 : DIM * (DIM + 1) / 2)
+                                             : (DIM == 2 ? 9 : 19);
     const gptr<const float> gp = geo_p + e * ngp * S::ND + X.p;
-    if (PHYS == SEM_ACOUSTIC) {
+    if (PHYS == SEM_ACOUSTIC && GM == kGeoQuadRef) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) M.Jinv[k] = gp[k * S::ND];
+      M.wdet = gp[4 * S::ND];  // w det J vp^2
+    } else if (PHYS == SEM_ACOUSTIC) {
 #pragma unroll
       for (int k = 0; k < ngp; ++k) M.G[k] = gp[k * S::ND];
     } else {
@@ -362,11 +396,23 @@ __device__ __forceinline__ void load_mat(const NodeCtx<DIM, N> &X, int64_t e, gp
 }
 
 // Stage 1: fluxes of this node into LDS.  su: C x ND field, sF: C x DIM x ND.
-template <int DIM, int N, int PHYS>
+// QREF (reference quad operator): the physical-axis fluxes times w det J,
+// without J^-T (div_ref applies J^-1 at the output node).
+template <int DIM, int N, int PHYS, bool QREF = false>
 __device__ __forceinline__ void node_flux(const NodeCtx<DIM, N> &X, const NodeMat<DIM, PHYS> &M,
                                           const float *su, float *sF) {
   using S = Sh<DIM, N>;
-  if (PHYS == SEM_ACOUSTIC) {
+  if (PHYS == SEM_ACOUSTIC && QREF) {
+    float g[3];
+    X.grad(su, g);
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      float d = 0.f;
+#pragma unroll
+      for (int b = 0; b < DIM; ++b) d = fmaf(M.Jinv[a * DIM + b], g[b], d);
+      sF[a * S::ND + X.p] = M.wdet * d;  // Scalar.cpp:45-71: w det J vp^2 grad u
+    }
+  } else if (PHYS == SEM_ACOUSTIC) {
     float g[3];
     X.grad(su, g);
     if (DIM == 2) {
@@ -410,6 +456,14 @@ __device__ __forceinline__ void node_flux(const NodeCtx<DIM, N> &X, const NodeMa
       sig[0][2] = sig[2][0] = M.c[7] * e4;
       sig[0][1] = sig[1][0] = M.c[8] * e5;
     }
+    if (QREF) {
+      // rows (sigma_xx, sigma_xy) and (sigma_xy, sigma_yy), Elastic2D.cpp:84-107
+#pragma unroll
+      for (int c = 0; c < DIM; ++c)
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) sF[(c * DIM + a) * S::ND + X.p] = sig[c][a];
+      return;
+    }
     // F_{c,a} = w detJ sum_b Jinv[b][a] sigma_{c,b}
 #pragma unroll
     for (int c = 0; c < DIM; ++c)
@@ -439,10 +493,11 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
   s = t;
 }
 
-template <int DIM, int N, int PHYS, bool AFFINE>
+template <int DIM, int N, int PHYS, int GM>
 __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
   using S = Sh<DIM, N>;
   constexpr int C = PHYS == SEM_ACOUSTIC ? 1 : DIM;
+  constexpr bool QREF = GM == kGeoQuadRef;
   extern __shared__ float lds[];
   // XCD-aware item order: consecutive hardware block ids round-robin over
   // 8 XCDs; give each XCD a contiguous range of items.
@@ -473,7 +528,7 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
     X.init(p);
     word = glb(G.conn)[e * S::ND + p];
     g = word & kNodeMask;
-    load_mat<DIM, N, PHYS, AFFINE>(X, e, glb(G.geo_e), glb(G.par_e), glb(G.geo_p), M);
+    load_mat<DIM, N, PHYS, GM>(X, e, glb(G.geo_e), glb(G.par_e), glb(G.geo_p), M);
     if (word & kFlagLast) mi = glb(G.mi)[g];
   }
   const int Sg = G.S;
@@ -497,7 +552,7 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
       }
     }
     __syncthreads();
-    if (active) node_flux<DIM, N, PHYS>(X, M, su, sF);
+    if (active) node_flux<DIM, N, PHYS, QREF>(X, M, su, sF);
     __syncthreads();
     if (active) {
       float nu[C];
@@ -507,7 +562,7 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
       float f[C];
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        float ku = X.div(sF + c * DIM * S::ND);
+        float ku = QREF ? X.div_ref(sF + c * DIM * S::ND, M.Jinv) : X.div(sF + c * DIM * S::ND);
         f[c] = (word & kFlagDirichlet) ? 0.f : -ku;
       }
       for (int k = 0; k < G.nsrc; ++k) {
@@ -526,11 +581,12 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
         else f[c] = t;
       }
       if (fin) {
-        // Order2Newmark.cpp:68-115; with the 3-D quirk the list is {x, y, x}
-        const bool q3 = (C == 3) && (G.quirks & SEM_QUIRK_NEWMARK3D);
+        // Order2Newmark.cpp:68-115; with the field-list quirk every elastic
+        // mesh advances x twice, y once and (3-D) z never
+        const bool q3 = (C >= 2) && (G.quirks & SEM_QUIRK_NEWMARK3D);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-          if (q3 && c == 2) continue;
+          if (q3 && c == 2) continue;  // c == 2 exists only in 3-D
           const int64_t idx = ((int64_t)s * C + c) * G.G + g;
           float a = mi * f[c];
           float ap = pap[c];
@@ -577,10 +633,11 @@ __global__ void k_plain_gather(int64_t E, int nd, int C, int64_t G, const uint32
 
 
 
-template <int DIM, int N, int PHYS>
+template <int DIM, int N, int PHYS, int GM>
 __global__ void __launch_bounds__(512) k_plain_stiffness(PlainArgs A) {
   using S = Sh<DIM, N>;
   constexpr int C = PHYS == SEM_ACOUSTIC ? 1 : DIM;
+  constexpr bool QREF = GM == kGeoQuadRef;
   extern __shared__ float lds[];
   const int t = threadIdx.x;
   const int el = t / S::ND;
@@ -593,13 +650,13 @@ __global__ void __launch_bounds__(512) k_plain_stiffness(PlainArgs A) {
   NodeMat<DIM, PHYS> M;
   if (active) {
     X.init(p);
-    load_mat<DIM, N, PHYS, false>(X, e, glb((const float *)nullptr), glb((const float *)nullptr),
-                                  glb(A.geo_p), M);
+    load_mat<DIM, N, PHYS, GM>(X, e, glb((const float *)nullptr), glb((const float *)nullptr),
+                               glb(A.geo_p), M);
 #pragma unroll
     for (int c = 0; c < C; ++c) su[c * S::ND + p] = glb(A.uloc)[(e * C + c) * S::ND + p];
   }
   __syncthreads();
-  if (active) node_flux<DIM, N, PHYS>(X, M, su, sF);
+  if (active) node_flux<DIM, N, PHYS, QREF>(X, M, su, sF);
   __syncthreads();
   if (!active) return;
   const uint32_t word = glb(A.conn)[e * S::ND + p];
@@ -607,7 +664,7 @@ __global__ void __launch_bounds__(512) k_plain_stiffness(PlainArgs A) {
   float f[C];
 #pragma unroll
   for (int c = 0; c < C; ++c) {
-    float ku = X.div(sF + c * DIM * S::ND);
+    float ku = QREF ? X.div_ref(sF + c * DIM * S::ND, M.Jinv) : X.div(sF + c * DIM * S::ND);
     f[c] = (word & kFlagDirichlet) ? 0.f : -ku;
   }
   for (int k = 0; k < A.nsrc; ++k) {
@@ -639,7 +696,7 @@ __global__ void k_plain_update(int64_t G, int C, int quirks, float dt, float acl
                                uint32_t *__restrict__ kc) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= G) return;
-  const bool q3 = C == 3 && (quirks & SEM_QUIRK_NEWMARK3D);
+  const bool q3 = C >= 2 && (quirks & SEM_QUIRK_NEWMARK3D);  // {x, y, x}: x twice, z never
   float m = mi[g];
   for (int c = 0; c < C; ++c) {
     if (q3 && c == 2) continue;
@@ -737,12 +794,12 @@ int launch_affine_geometry(int dim, int physics, int64_t E, const double *vtx, f
 #define SEM_ORDERS_3D(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
 #define SEM_ORDERS_2D(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
 
-int launch_point_geometry(int dim, int order, int physics, int64_t E, const double *vtx,
+int launch_point_geometry(int dim, int order, int physics, int gmode, int64_t E, const double *vtx,
                           const double *vpar, int npar, float *geo_p, double *me, hipStream_t st) {
   const int n = order + 1;
   const int64_t total = E * (dim == 2 ? n * n : n * n * n);
-#define PG3(N) case N: k_point_geometry<3, N><<<nblocks(total, 128), 128, 0, st>>>(physics, E, vtx, vpar, npar, geo_p, me); break;
-#define PG2(N) case N: k_point_geometry<2, N><<<nblocks(total, 128), 128, 0, st>>>(physics, E, vtx, vpar, npar, geo_p, me); break;
+#define PG3(N) case N: k_point_geometry<3, N><<<nblocks(total, 128), 128, 0, st>>>(physics, gmode, E, vtx, vpar, npar, geo_p, me); break;
+#define PG2(N) case N: k_point_geometry<2, N><<<nblocks(total, 128), 128, 0, st>>>(physics, gmode, E, vtx, vpar, npar, geo_p, me); break;
   if (dim == 3) {
     switch (order) { SEM_ORDERS_3D(PG3) default: set_error("order not compiled"); return -1; }
   } else {
@@ -787,20 +844,24 @@ int launch_colour_count(int64_t G, int32_t *hits, unsigned long long *viol, hipS
   return last_launch("colour count");
 }
 
-int launch_fused(int dim, int order, int physics, bool affine, int nitems, int epb, int nd, int c,
+int launch_fused(int dim, int order, int physics, int gmode, int nitems, int epb, int nd, int c,
                  const FusedArgs &a, hipStream_t st) {
   if (nitems <= 0) return 0;
   const int bt = block_threads(nd, epb);
   if (bt > 512) { set_error("elements per block too large"); return -1; }
   const size_t shm = fused_lds_bytes(dim, nd, c, epb);
   dim3 grid(nitems), block(bt);
-#define FU(D, N, P, AF) hipLaunchKernelGGL((k_fused<D, N, P, AF>), grid, block, shm, st, a)
+#define FU(D, N, P, GMV) hipLaunchKernelGGL((k_fused<D, N, P, GMV>), grid, block, shm, st, a)
+#define FUP(D, N, P)                                                          \
+  if (gmode == kGeoAffine) FU(D, N, P, kGeoAffine);                           \
+  else if (gmode == kGeoQuadRef) FU(D, N, P, (D == 2 ? kGeoQuadRef : kGeoPoint)); \
+  else FU(D, N, P, kGeoPoint);
 #define FU4(D, N)                                      \
   case N:                                              \
     if (physics == SEM_ACOUSTIC) {                     \
-      if (affine) FU(D, N, 0, true); else FU(D, N, 0, false); \
+      FUP(D, N, 0)                                     \
     } else {                                           \
-      if (affine) FU(D, N, 1, true); else FU(D, N, 1, false); \
+      FUP(D, N, 1)                                     \
     }                                                  \
     break;
 #define FU3(N) FU4(3, N)
@@ -825,14 +886,20 @@ int launch_plain_gather(int64_t E, int nd, int C, int64_t G, const uint32_t *con
   return last_launch("plain gather");
 }
 
-int launch_plain_stiffness(int dim, int order, int physics, int64_t E, int nd, int c,
+int launch_plain_stiffness(int dim, int order, int physics, int gmode, int64_t E, int nd, int c,
                            const PlainArgs &a, hipStream_t st) {
   const int bt = block_threads(nd, a.epb);
   const size_t shm = fused_lds_bytes(dim, nd, c, a.epb);
   dim3 grid(nblocks(E, a.epb)), block(bt);
+  // the plain path always takes pointwise factors
+  const bool qref = dim == 2 && gmode == kGeoQuadRef;
+#define PSG(D, N, P)                                                                            \
+  if (qref) hipLaunchKernelGGL((k_plain_stiffness<D, N, P, (D == 2 ? kGeoQuadRef : kGeoPoint)>), \
+                               grid, block, shm, st, a);                                         \
+  else hipLaunchKernelGGL((k_plain_stiffness<D, N, P, kGeoPoint>), grid, block, shm, st, a);
 #define PS(D, N) case N: \
-    if (physics == SEM_ACOUSTIC) hipLaunchKernelGGL((k_plain_stiffness<D, N, 0>), grid, block, shm, st, a); \
-    else hipLaunchKernelGGL((k_plain_stiffness<D, N, 1>), grid, block, shm, st, a); \
+    if (physics == SEM_ACOUSTIC) { PSG(D, N, 0) } \
+    else { PSG(D, N, 1) } \
     break;
 #define PS3(N) PS(3, N)
 #define PS2(N) PS(2, N)

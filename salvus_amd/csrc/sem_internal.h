@@ -106,8 +106,14 @@ __device__ __forceinline__ float comp_v(uint32_t w) { return __uint_as_float(w &
 __host__ __device__ inline int affine_geo_count(int dim, int physics) {
   return physics == SEM_ACOUSTIC ? dim * (dim + 1) / 2 : dim * dim + 1;
 }
-__host__ __device__ inline int point_geo_count(int dim, int physics) {
-  if (physics == SEM_ACOUSTIC) return dim * (dim + 1) / 2;
+// Geometry modes of the element kernels: pointwise factors, per-element
+// affine factors, or pointwise factors for the reference's quad operator
+// (SEM_QUIRK_QUAD_JACOBIAN on non-affine quads: J^-1 and w det J of the
+// QuadP1.h:20-27 Jacobian per node, J^-1 applied at the output node).
+enum { kGeoPoint = 0, kGeoAffine = 1, kGeoQuadRef = 2 };
+// With kGeoQuadRef the acoustic factors are Jinv(4) + w det J vp^2 (1).
+__host__ __device__ inline int point_geo_count(int dim, int physics, int gmode = kGeoPoint) {
+  if (physics == SEM_ACOUSTIC) return gmode == kGeoQuadRef ? 5 : dim * (dim + 1) / 2;
   return dim == 2 ? 9 : 19;
 }
 // Vertex material the fused kernels interpolate on the fly (stiffness only):
@@ -242,7 +248,7 @@ struct RecArgs {
 int init_device_tables(int device);
 int launch_affine_geometry(int dim, int physics, int64_t E, const double *vtx, float *geo,
                            hipStream_t st);
-int launch_point_geometry(int dim, int order, int physics, int64_t E, const double *vtx,
+int launch_point_geometry(int dim, int order, int physics, int gmode, int64_t E, const double *vtx,
                           const double *vpar, int npar, float *geo_p, double *me, hipStream_t st);
 int launch_mass_scatter(int64_t first, int64_t count, int nd, const uint32_t *conn,
                         const double *me, double *M, hipStream_t st);
@@ -258,12 +264,12 @@ inline int block_threads(int nd, int epb) { return ((nd * epb + 63) / 64) * 64; 
 inline size_t fused_lds_bytes(int dim, int nd, int c, int epb) {
   return sizeof(float) * (size_t)epb * c * nd * (1 + dim);
 }
-int launch_fused(int dim, int order, int physics, bool affine, int nitems, int epb, int nd, int c,
+int launch_fused(int dim, int order, int physics, int gmode, int nitems, int epb, int nd, int c,
                  const FusedArgs &a, hipStream_t st);
 int launch_permute_f32(int64_t n, const int32_t *to, const float *src, float *dst, hipStream_t st);
 int launch_plain_gather(int64_t E, int nd, int C, int64_t G, const uint32_t *conn, const float *u,
                         float *uloc, hipStream_t st);
-int launch_plain_stiffness(int dim, int order, int physics, int64_t E, int nd, int c,
+int launch_plain_stiffness(int dim, int order, int physics, int gmode, int64_t E, int nd, int c,
                            const PlainArgs &a, hipStream_t st);
 int launch_plain_scatter(int64_t first, int64_t count, int nd, int C, int64_t G,
                          const uint32_t *conn, const float *aloc, float *acc, hipStream_t st);
@@ -332,6 +338,16 @@ struct Group;
 struct sem_batch;
 
 namespace sem {
+// Geometry mode of a mesh's element kernels (kGeo*).
+inline int geo_mode(const sem_mesh *m) {
+  if (m->affine) return kGeoAffine;
+  return (m->dim == 2 && (m->quirks & SEM_QUIRK_QUAD_JACOBIAN)) ? kGeoQuadRef : kGeoPoint;
+}
+// Mode of the pointwise factors (plain path, mass): the reference quad
+// operator where it differs from the consistent one, else kGeoPoint.
+inline int pointwise_mode(const sem_mesh *m) {
+  return geo_mode(m) == kGeoQuadRef ? kGeoQuadRef : kGeoPoint;
+}
 // Tiles a logically Cartesian mesh; 0 = tiled, 1 = not eligible.
 int build_tiles(sem_mesh *m, int Tx, int Ty, int Tz, TileSet &ts);
 }  // namespace sem

@@ -292,7 +292,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
   const gptr<float> gacc = glb(G.acc);
   const gptr<const float> gmi = glb(G.mi);
   const uint32_t GG = (uint32_t)G.G, Sg = (uint32_t)G.S;
-  const bool q3 = (C == 3) && (G.quirks & SEM_QUIRK_NEWMARK3D);
+  const bool q3 = (C >= 2) && (G.quirks & SEM_QUIRK_NEWMARK3D);  // {x, y, x}: x twice, z never
 
   __syncthreads();  // tile data and the first shot's lattice staged
   const int nts = sS[4 * kTileMaxSrc];

@@ -21,8 +21,8 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libsalvus_sem.so"
 
 ACOUSTIC, ELASTIC = 0, 1
-QUIRK_NEWMARK3D, QUIRK_E3D_SOURCE = 1, 2
-DEFAULT_QUIRKS = QUIRK_E3D_SOURCE
+QUIRK_NEWMARK3D, QUIRK_E3D_SOURCE, QUIRK_QUAD_JACOBIAN = 1, 2, 4
+DEFAULT_QUIRKS = QUIRK_E3D_SOURCE | QUIRK_QUAD_JACOBIAN
 PATH_PLAIN, PATH_FUSED = 0, 1
 FIELD = {"u": 0, "v": 1, "a_": 2}
 

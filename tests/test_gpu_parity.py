@@ -48,26 +48,44 @@ def test_mesh_maps_bit_exact(dim, physics):
     assert np.max(np.abs(mi - ref) / np.abs(ref)) < 1e-6
 
 
+# Non-parallelogram quads run the reference's quad operator (QuadP1.h:20-27
+# Jacobian, J^-1 at the output node, TensorQuad.cpp:383-390) by default and
+# the consistent one with the switch off; the oracle restates both.
+QUAD_MODES = [sem.DEFAULT_QUIRKS, sem.DEFAULT_QUIRKS & ~sem.QUIRK_QUAD_JACOBIAN]
+
+
 @pytest.mark.parametrize("dim,physics", CASES)
 @pytest.mark.parametrize("order", [3, 4])
 @pytest.mark.parametrize("distorted", [False, True])
 def test_stiffness_operator(dim, physics, order, distorted):
-    if distorted and dim == 2:
-        pytest.skip("non-affine quads: reference applies J^-1 at output nodes "
-                    "(TensorQuad.cpp:383-390) with a swapped-coordinate Jacobian (QuadP1.h:20-27); "
-                    "product uses the consistent form")
     P = pb.make(dim, physics, 4 if dim == 3 else 7, order=order, seed=order, distorted=distorted)
-    M = mesh_for(P)
-    S = oracle_for(P)
-    assert M.affine == (not distorted)
-    rng = np.random.default_rng(11)
-    xyz = S.node_coords()
-    u = np.stack([np.sin(0.05 * xyz @ rng.normal(size=dim)) + 0.1 * rng.normal(size=S.G)
-                  for _ in range(S.c)])
-    ref = S.apply_stiffness(u)
-    for path in (sem.PATH_FUSED, sem.PATH_PLAIN):
-        ku = M.apply_stiffness(u, path).astype(np.float64)
-        assert pb.rel_l2(ku, ref) < TOL_OP, (path, pb.rel_l2(ku, ref))
+    for quirks in (QUAD_MODES if dim == 2 else QUAD_MODES[:1]):
+        M = mesh_for(P, quirks)
+        S = oracle_for(P, quirks)
+        assert M.affine == (not distorted)
+        rng = np.random.default_rng(11)
+        xyz = S.node_coords()
+        u = np.stack([np.sin(0.05 * xyz @ rng.normal(size=dim)) + 0.1 * rng.normal(size=S.G)
+                      for _ in range(S.c)])
+        ref = S.apply_stiffness(u)
+        mi = M.inverse_mass().astype(np.float64)
+        assert np.max(np.abs(mi - S.inverse_mass()) / S.inverse_mass()) < 1e-6
+        for path in (sem.PATH_FUSED, sem.PATH_PLAIN):
+            ku = M.apply_stiffness(u, path).astype(np.float64)
+            assert pb.rel_l2(ku, ref) < TOL_OP, (quirks, path, pb.rel_l2(ku, ref))
+
+
+@pytest.mark.parametrize("physics", [0, 1])
+@pytest.mark.parametrize("quirks", QUAD_MODES)
+@pytest.mark.parametrize("path", [sem.PATH_FUSED, sem.PATH_PLAIN])
+def test_distorted_quad_run(physics, quirks, path):
+    """A run on non-parallelogram quads in both quad-operator modes."""
+    P = pb.make(2, physics, 10, order=4, seed=8, distorted=True, nsteps=250)
+    seis, seis_ref, u, u_ref, _, _ = _run_both(P, path, quirks=quirks)
+    e_s, e_u = pb.rel_l2(seis, seis_ref), pb.rel_l2(u, u_ref)
+    print("distorted quads physics %d quirks %d path %d: receivers %.3g, u %.3g"
+          % (physics, quirks, path, e_s, e_u))
+    assert e_s < TOL_RUN and e_u < TOL_RUN
 
 
 def _run_both(P, path, quirks=po.DEFAULT_QUIRKS, init=None):
@@ -109,12 +127,17 @@ def test_distorted_hex_run():
     assert pb.rel_l2(seis, seis_ref) < TOL_RUN and pb.rel_l2(u, u_ref) < TOL_RUN
 
 
-def test_newmark3d_quirk_matches_oracle():
-    P = pb.make(3, 1, 4, order=3, seed=6, nsteps=120)
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("path", [sem.PATH_FUSED, sem.PATH_PLAIN])
+def test_newmark_quirk_matches_oracle(dim, path):
+    """Order2Newmark.cpp:93-96's list {ax, ay, ax, a} on Elastic2D and
+    Elastic3D: x advanced twice per step, y once, z never."""
+    P = pb.make(dim, 1, 4 if dim == 3 else 8, order=3, seed=6, nsteps=120)
     q = po.QUIRK_NEWMARK3D | po.QUIRK_E3D_SOURCE
-    seis, seis_ref, u, u_ref, _, _ = _run_both(P, sem.PATH_FUSED, quirks=q)
-    assert pb.rel_l2(u, u_ref) < TOL_RUN
-    assert np.abs(u[2]).max() == 0.0  # z never advances under the reference's list
+    seis, seis_ref, u, u_ref, _, _ = _run_both(P, path, quirks=q)
+    assert pb.rel_l2(u, u_ref) < TOL_RUN and pb.rel_l2(seis, seis_ref) < TOL_RUN
+    if dim == 3:
+        assert np.abs(u[2]).max() == 0.0  # z never advances under the reference's list
 
 
 def test_quad_eigenfunction_on_gpu():

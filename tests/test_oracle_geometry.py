@@ -87,3 +87,35 @@ def test_hex_jacobian_is_map_derivative():
                   (x(r, s, t + h) - x(r, s, t - h)) / (2 * h)])
     np.testing.assert_allclose(inv, np.linalg.inv(J), rtol=1e-7)
     assert det == pytest.approx(np.linalg.det(J), rel=1e-8)
+
+
+# a trapezoid: not a parallelogram, so the two quad Jacobians differ inside it
+TRAPEZOID = np.array([[0.0, 0.0], [4.0, 0.0], [3.0, 2.5], [0.5, 2.0]])
+
+
+def test_quad_consistent_jacobian_is_the_p1_derivative():
+    """The consistent Jacobian is d x / d(r, s) of the P1 map (QuadP1.h:98-108
+    weights times the vertices), checked by central differences; the
+    reference's rows (QuadP1.h:20-27) agree with it on parallelograms and at
+    the centre only."""
+    def x_of(r, s):
+        return po.interp((r, s)) @ TRAPEZOID
+
+    h = 1e-6
+    for r, s in ((0.3, -0.7), (-0.9, 0.8), (0.0, 0.0), (1.0, -1.0)):
+        J = np.array([(x_of(r + h, s) - x_of(r - h, s)) / (2 * h),
+                      (x_of(r, s + h) - x_of(r, s - h)) / (2 * h)])
+        det, inv = po.quad_invjac_consistent(r, s, TRAPEZOID)
+        assert det == pytest.approx(np.linalg.det(J), rel=1e-9)
+        np.testing.assert_allclose(inv, np.linalg.inv(J), rtol=1e-8)
+    det_c, inv_c = po.quad_invjac_consistent(0.0, 0.0, TRAPEZOID)
+    det_r, inv_r = po.quad_invjac(0.0, 0.0, TRAPEZOID)
+    assert det_c == det_r
+    assert po.quad_invjac(0.5, -0.5, TRAPEZOID)[0] != pytest.approx(
+        po.quad_invjac_consistent(0.5, -0.5, TRAPEZOID)[0], rel=1e-6)
+    para = np.array([[0.0, 0.0], [4.0, 1.0], [5.0, 3.0], [1.0, 2.0]])
+    for r, s in ((0.3, -0.7), (-0.9, 0.8)):
+        dr_, ir_ = po.quad_invjac(r, s, para)
+        dc_, ic_ = po.quad_invjac_consistent(r, s, para)
+        assert dr_ == pytest.approx(dc_, rel=1e-15)
+        np.testing.assert_allclose(ir_, ic_, rtol=1e-14)
