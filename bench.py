@@ -193,6 +193,10 @@ def main():
               "--gpus %d is not supported (use --gpus 1)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    import math
+    # graph blocks that divide both the warm-up and the timed steps, so the
+    # timed region replays graphs captured during the warm-up
+    os.environ["SEM_GRAPH_STEPS"] = str(min(50, math.gcd(args.warmup, args.steps) or 50))
     from salvus_amd import sem, workloads
 
     timing_steps = 10

@@ -40,12 +40,17 @@ void free_group(Group &g) {
 void drop_graph(sem_batch *b) {
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   if (b->graph_def) (void)hipGraphDestroy(b->graph_def);
+  if (b->tail) (void)hipGraphExecDestroy(b->tail);
+  if (b->tail_def) (void)hipGraphDestroy(b->tail_def);
   b->graph = nullptr;
   b->graph_def = nullptr;
+  b->tail = nullptr;
+  b->tail_def = nullptr;
+  b->tail_steps = 0;
 }
 
 // Default launch shape: elements per block so a block has ~128-256 threads.
-static int default_epb(int nd) {
+int default_epb(int nd) {
   int best = 1;
   for (int epb = 1; epb * nd <= 512; ++epb)
     if (block_threads(nd, epb) <= 256) best = epb;
@@ -402,6 +407,9 @@ int finalize_groups(sem_batch *b, int epb_hint) {
     if (b->groups.empty()) min_steps = 1;
     b->block_steps = (int)std::min<int64_t>(std::max<int64_t>(1, 4096 / std::max<int>(1, (int)b->launches.size())), 50);
     b->block_steps = std::max(1, std::min<int>(b->block_steps, (int)min_steps));
+    // SEM_GRAPH_STEPS=k: steps per captured graph block (1..50)
+    const char *gs = getenv("SEM_GRAPH_STEPS");
+    if (gs && atoi(gs) > 0) b->block_steps = std::max(1, std::min(atoi(gs), 50));
   }
   HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
@@ -501,12 +509,20 @@ int enqueue_plain_step(sem_batch *b) {
   return 0;
 }
 
-int capture_graph(sem_batch *b) {
-  if (b->graph) return 0;
+int capture_graph(sem_batch *b, int steps) {
+  const bool full = steps == b->block_steps;
+  if (full && b->graph) return 0;
+  if (!full && b->tail && b->tail_steps == steps) return 0;
+  hipGraphExec_t &exec = full ? b->graph : b->tail;
+  hipGraph_t &def = full ? b->graph_def : b->tail_def;
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (def) (void)hipGraphDestroy(def);
+  exec = nullptr;
+  def = nullptr;
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
-  for (int k = 0; k < b->block_steps && !rc; ++k) rc = enqueue_fused_step(b, k, 0);
-  if (!rc) rc = launch_advance((int)b->groups.size(), b->d_groups, b->block_steps, b->stream);
+  for (int k = 0; k < steps && !rc; ++k) rc = enqueue_fused_step(b, k, 0);
+  if (!rc) rc = launch_advance((int)b->groups.size(), b->d_groups, steps, b->stream);
   hipGraph_t gdef = nullptr;
   hipError_t e = hipStreamEndCapture(b->stream, &gdef);
   if (rc) {
@@ -514,8 +530,9 @@ int capture_graph(sem_batch *b) {
     return rc;
   }
   HIPCHK(e);
-  b->graph_def = gdef;
-  HIPCHK(hipGraphInstantiate(&b->graph, gdef, nullptr, nullptr, 0));
+  def = gdef;
+  HIPCHK(hipGraphInstantiate(&exec, gdef, nullptr, nullptr, 0));
+  if (!full) b->tail_steps = steps;
   return 0;
 }
 
@@ -839,10 +856,11 @@ int sem_batch_run(sem_batch *b, int64_t nsteps) {
     const char *ng = getenv("SEM_NO_GRAPH");
     const bool graphs = !(ng && ng[0] == '1') && !b->timing;
     while (left > 0) {
-      if (graphs && left >= b->block_steps) {
-        SEM_TRY(capture_graph(b));
-        HIPCHK(hipGraphLaunch(b->graph, b->stream));
-        left -= b->block_steps;
+      if (graphs) {  // full blocks, then one (cached) tail block
+        const int k = (int)std::min<int64_t>(left, b->block_steps);
+        SEM_TRY(capture_graph(b, k));
+        HIPCHK(hipGraphLaunch(k == b->block_steps ? b->graph : b->tail, b->stream));
+        left -= k;
       } else {
         const int k = (int)std::min<int64_t>(left, b->block_steps);
         for (int j = 0; j < k; ++j) SEM_TRY(enqueue_fused_step(b, j, 0));

@@ -108,8 +108,11 @@ struct sem_batch {
   std::vector<sem::RecSet> recsets;
   float *d_uloc = nullptr, *d_aloc = nullptr;  // plain path scratch
   size_t uloc_cap = 0;
-  hipGraphExec_t graph = nullptr;
+  hipGraphExec_t graph = nullptr;   // block_steps steps
   hipGraph_t graph_def = nullptr;
+  hipGraphExec_t tail = nullptr;    // tail_steps (< block_steps) steps
+  hipGraph_t tail_def = nullptr;
+  int tail_steps = 0;
   int64_t steps_done = 0;
   bool timing = false;
   double kernel_ms = 0;
@@ -142,7 +145,11 @@ void free_group(Group &g);
 int finalize_groups(sem_batch *b, int epb_hint);
 void drop_graph(sem_batch *b);
 int enqueue_fused_step(sem_batch *b, int koff, int ku_only);
+// Default elements per block of the element kernel for nd nodes per element.
+int default_epb(int nd);
 int enqueue_plain_step(sem_batch *b);
-int capture_graph(sem_batch *b);
+// Captures (once) the graph of k steps: k == block_steps, or one cached
+// shorter tail block.
+int capture_graph(sem_batch *b, int k);
 
 }  // namespace sem

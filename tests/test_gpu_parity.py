@@ -192,9 +192,14 @@ def test_batched_identical_to_alone():
         np.testing.assert_array_equal(A.get_field(a, "u"), B.get_field(k, "u"))
 
 
-def test_graph_replay_identical_to_eager(monkeypatch):
+@pytest.mark.parametrize("block", ["", "7"])
+def test_graph_replay_identical_to_eager(monkeypatch, block):
+    """Runs of 40 steps: with the default block (50) each is one captured
+    40-step tail graph; with blocks of 7, five full-block replays and a
+    5-step tail."""
     P = pb.make(2, 1, 8, order=4, seed=9, nsteps=160)
     out = []
+    monkeypatch.setenv("SEM_GRAPH_STEPS", block)
     for no_graph in ("1", "0", "0"):
         monkeypatch.setenv("SEM_NO_GRAPH", no_graph)
         M = mesh_for(P)
