@@ -115,6 +115,20 @@ int sem_mesh_set_dirichlet(sem_mesh *mesh, const uint8_t *node_mask);
  * through the given path's kernels (Problem.cpp:176-208 without source and
  * mass); u and ku are host buffers. */
 int sem_mesh_apply_stiffness(sem_mesh *mesh, int path, const float *u, float *ku);
+/* Allocation-free variant of sem_mesh_apply_stiffness (no counterpart in
+ * the reference; for callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops).  u and ku are DEVICE buffers of
+ * c x G floats in the DMPlex DOF order; ku = K u.  It allocates nothing: its
+ * group descriptor, step counter and work items live in a 4 MiB device array
+ * of the library, which each GPU holds until the process ends; a mesh whose
+ * largest element colour needs more work items than that array holds
+ * (262,112 items of up to 256 threads' worth of elements: at order 4 in 3-D
+ * two elements each, 524,224 elements per colour) is rejected with an
+ * error.  Runs the element-colour kernel (also on meshes with tiles)
+ * on `stream` (a hipStream_t; NULL = the mesh's stream, and the call then
+ * waits for the result).  Calls that overlap on one GPU share that array and
+ * race: serialise them. */
+int sem_mesh_apply_stiffness_noalloc(sem_mesh *mesh, const float *u, float *ku, void *stream);
 /* Race check on the device: returns in *violations the number of
  * (colour, node) pairs touched by more than one element of that colour. */
 int sem_mesh_check_colouring(sem_mesh *mesh, int64_t *violations);
