@@ -39,7 +39,7 @@ EXPORTS = (
     "sem_reference_step_count", "sem_server_create", "sem_server_destroy", "sem_server_clock",
     "sem_server_submit", "sem_server_pump", "sem_server_request", "sem_server_result",
     "sem_mesh_autotune", "sem_shot_add_file_source", "sem_receivers_write_h5",
-    "sem_source_read_h5", "sem_movie_write_h5",
+    "sem_source_read_h5", "sem_movie_write_h5", "sem_mesh_apply_stiffness_noalloc",
 )
 SOURCE_DOUBLES = 19
 
@@ -76,6 +76,7 @@ def lib():
         L.sem_mesh_set_dirichlet.argtypes = [_VP, C.POINTER(C.c_uint8)]
         L.sem_mesh_apply_stiffness.argtypes = [_VP, C.c_int, _F, _F]
         L.sem_mesh_check_colouring.argtypes = [_VP, _I64]
+        L.sem_mesh_apply_stiffness_noalloc.argtypes = [_VP, C.c_void_p, C.c_void_p, C.c_void_p]
         L.sem_batch_create.argtypes = [C.POINTER(_VP), C.c_int]
         L.sem_batch_destroy.argtypes = [_VP]
         L.sem_batch_add_shot.argtypes = [_VP, _VP, C.c_int64, C.c_double, C.POINTER(C.c_int)]
