@@ -83,10 +83,15 @@ int setup_group(sem_batch *b, int gi) {
   const size_t len = (size_t)m->c * m->G * g.S;
   SEM_TRY(dalloc(&g.fields, 5 * len, &g.bytes));
   std::vector<float> buf;
-  for (int which = 0; which < 5; ++which) {  // u v ap kc acc; kc = 0 is two bf16 zeros
+  // u, w, a_, kc, acc; kc = 0 is two bf16 zeros; w = v + dt/2 a_ from the
+  // shot's initial v and a_
+  const double half_dt = 0.5 * g.dt;
+  for (int which = 0; which < 5; ++which) {
     bool any = false;
-    if (which < 3)
-      for (int sl = 0; sl < g.S; ++sl) any |= !b->shots[g.shots[sl]].init[which].empty();
+    for (int sl = 0; sl < g.S && which < 3; ++sl) {
+      const Shot &s = b->shots[g.shots[sl]];
+      any |= !s.init[which].empty() || (which == 1 && !s.init[2].empty());
+    }
     if (!any) {  // zero start: a fill kernel, no host buffer
       SEM_TRY(launch_fill_f32((int64_t)len, g.fields + which * len, 0.f, b->stream));
       continue;
@@ -94,8 +99,16 @@ int setup_group(sem_batch *b, int gi) {
     buf.assign(len, 0.f);
     for (int sl = 0; sl < g.S; ++sl) {
       const Shot &s = b->shots[g.shots[sl]];
-      if (s.init[which].empty()) continue;
       const size_t cg = (size_t)m->c * m->G;
+      const std::vector<double> &v0 = s.init[1], &a0 = s.init[2];
+      if (which == 1) {
+        if (v0.empty() && a0.empty()) continue;
+        for (size_t i = 0; i < cg; ++i)
+          buf[sl * cg + field_pos(g, i)] =
+              (float)((v0.empty() ? 0.0 : v0[i]) + (a0.empty() ? 0.0 : half_dt * a0[i]));
+        continue;
+      }
+      if (s.init[which].empty()) continue;
       for (size_t i = 0; i < cg; ++i) buf[sl * cg + field_pos(g, i)] = (float)s.init[which][i];
     }
     HIPCHK(hipMemcpyAsync(g.fields + which * len, buf.data(), len * 4, hipMemcpyHostToDevice, b->stream));
@@ -206,7 +219,7 @@ int setup_group(sem_batch *b, int gi) {
   h.geo_p = m->d_geo_p;
   h.mi = g.tiled ? m->d_mi_int : m->d_mi;
   h.u = g.fields;
-  h.v = g.fields + len;
+  h.w = g.fields + len;
   h.ap = g.fields + 2 * len;
   h.kc = reinterpret_cast<uint32_t *>(g.fields + 3 * len);
   h.acc = g.fields + 4 * len;
@@ -220,7 +233,6 @@ int setup_group(sem_batch *b, int gi) {
   h.step = g.d_step;
   h.dt = (float)g.dt;
   h.acl = (float)((1.0 / 2.0) * g.dt);           // Order2Newmark.cpp:89
-  h.dsp = (float)((1.0 / 2.0) * (g.dt * g.dt));  // Order2Newmark.cpp:90
   h.quirks = m->quirks;
   if ((int)b->h_groups.size() <= gi) b->h_groups.resize(gi + 1);
   b->h_groups[gi] = h;
@@ -501,8 +513,8 @@ int enqueue_plain_step(sem_batch *b) {
     for (int c = 0; c < m->ncol; ++c)
       SEM_TRY(launch_plain_scatter(m->col_start[c], m->col_start[c + 1] - m->col_start[c], m->nd, m->c,
                                    m->G, m->d_conn, b->d_aloc, h.acc, b->stream));
-    SEM_TRY(launch_plain_update(m->G, m->c, m->quirks, h.dt, h.acl, h.dsp, m->d_mi, h.acc, h.u, h.v,
-                                h.ap, h.kc, b->stream));
+    SEM_TRY(launch_plain_update(m->G, m->c, m->quirks, h.dt, m->d_mi, h.acc, h.u, h.w, h.ap, h.kc,
+                                b->stream));
   }
   SEM_TRY(enqueue_receivers(b, 0));
   SEM_TRY(launch_advance((int)b->groups.size(), b->d_groups, 1, b->stream));
@@ -727,15 +739,48 @@ int sem_shot_set_field(sem_batch *b, int shot, int which, const double *vals) {
     s->init[which].assign(vals, vals + len);
     return 0;
   }
-  // live update of this shot's slice
+  // live update of this shot's slice; v and a_ live in the state as
+  // w = v + dt/2 a_ and a_, so setting one keeps the other
   HIPCHK(hipSetDevice(b->device));
   Group &g = b->groups[s->group];
-  std::vector<float> buf(len);
-  float *base = field_base(b, *s, which) + (size_t)s->slot * len;
+  const double half_dt = 0.5 * g.dt;
+  std::vector<float> buf(len), wcur, acur;
+  auto fetch = [&](int w, std::vector<float> &out) {
+    out.resize(len);
+    HIPCHK(hipMemcpyAsync(out.data(), field_base(b, *s, w) + (size_t)s->slot * len, len * 4,
+                          hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+  };
+  if (which >= 1) {
+    SEM_TRY(fetch(1, wcur));
+    SEM_TRY(fetch(2, acur));
+  }
+  auto put = [&](int w) {
+    HIPCHK(hipMemcpyAsync(field_base(b, *s, w) + (size_t)s->slot * len, buf.data(), len * 4,
+                          hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+  };
+  if (which == 0) {
+    for (size_t i = 0; i < len; ++i) buf[field_pos(g, i)] = (float)vals[i];
+    return put(0);
+  }
+  if (which == 1) {  // w = v_new + dt/2 a_
+    for (size_t i = 0; i < len; ++i) {
+      const size_t k = field_pos(g, i);
+      buf[k] = (float)(vals[i] + half_dt * acur[k]);
+    }
+    return put(1);
+  }
+  // a_ new: w = (w - dt/2 a_old) + dt/2 a_new keeps v
+  for (size_t i = 0; i < len; ++i) {
+    const size_t k = field_pos(g, i);
+    buf[k] = (float)((double)wcur[k] - half_dt * acur[k] + half_dt * vals[i]);
+  }
+  SEM_TRY(put(1));
   for (size_t i = 0; i < len; ++i) buf[field_pos(g, i)] = (float)vals[i];
-  HIPCHK(hipMemcpyAsync(base, buf.data(), len * 4, hipMemcpyHostToDevice, b->stream));
-  HIPCHK(hipStreamSynchronize(b->stream));
-  return 0;
+  return put(2);
 }
 
 int sem_shot_get_field(sem_batch *b, int shot, int which, double *vals) {
@@ -749,11 +794,20 @@ int sem_shot_get_field(sem_batch *b, int shot, int which, double *vals) {
   }
   HIPCHK(hipSetDevice(b->device));
   Group &g = b->groups[s->group];
-  std::vector<float> buf(len);
+  std::vector<float> buf(len), ap;
   HIPCHK(hipMemcpyAsync(buf.data(), field_base(b, *s, which) + (size_t)s->slot * len, len * 4,
                         hipMemcpyDeviceToHost, b->stream));
+  if (which == 1) {  // v = w - dt/2 a_
+    ap.resize(len);
+    HIPCHK(hipMemcpyAsync(ap.data(), field_base(b, *s, 2) + (size_t)s->slot * len, len * 4,
+                          hipMemcpyDeviceToHost, b->stream));
+  }
   HIPCHK(hipStreamSynchronize(b->stream));
-  for (size_t i = 0; i < len; ++i) vals[i] = buf[field_pos(g, i)];
+  const double half_dt = 0.5 * g.dt;
+  for (size_t i = 0; i < len; ++i) {
+    const size_t k = field_pos(g, i);
+    vals[i] = which == 1 ? (double)buf[k] - half_dt * (double)ap[k] : (double)buf[k];
+  }
   return 0;
 }
 

@@ -509,7 +509,7 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
   const auto &G = glb(A.groups)[W.group];
   const int64_t step = *glb(G.step) + A.koff;
   if (step >= G.nsteps) return;  // uniform across the block
-  const gptr<float> gu = glb(G.u), gv = glb(G.v), gap = glb(G.ap), gacc = glb(G.acc);
+  const gptr<float> gu = glb(G.u), gw = glb(G.w), gap = glb(G.ap), gacc = glb(G.acc);
   const gptr<uint32_t> gkc = glb(G.kc);
 
   const int t = threadIdx.x;
@@ -538,7 +538,7 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
     // gather (shot 0 only: every later shot's u is loaded while the previous
     // shot integrates and finishes, and staged after this thread's own
     // finish), and this node's finish inputs loaded ahead of the element work
-    float pacc[C], pap[C], pv[C];
+    float pacc[C], pw[C];
     uint32_t pkc[C];
     if (active) {
 #pragma unroll
@@ -546,8 +546,7 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
         const int64_t idx = ((int64_t)s * C + c) * G.G + g;
         if (s == 0) su[c * S::ND + p] = gu[idx];
         pacc[c] = first ? 0.f : gacc[idx];
-        pap[c] = fin ? gap[idx] : 0.f;
-        pv[c] = fin ? gv[idx] : 0.f;
+        pw[c] = fin ? gw[idx] : 0.f;
         pkc[c] = fin ? gkc[idx] : 0u;
       }
     }
@@ -589,16 +588,14 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
           if (q3 && c == 2) continue;  // c == 2 exists only in 3-D
           const int64_t idx = ((int64_t)s * C + c) * G.G + g;
           float a = mi * f[c];
-          float ap = pap[c];
-          float v = pv[c], kv = comp_v(pkc[c]);
+          float w = pw[c], kw = comp_v(pkc[c]);
           float u = su[c * S::ND + p], ku = comp_u(pkc[c]);
           const int reps = (q3 && c == 0) ? 2 : 1;
-          for (int r = 0; r < reps; ++r) {
-            kahan_add(v, kv, G.acl * (a + ap));
-            kahan_add(u, ku, fmaf(G.dt, v, G.dsp * a));
-            ap = a;
+          for (int r = 0; r < reps; ++r) {  // w = v + dt/2 a_: w += dt a, u += dt w
+            kahan_add(w, kw, G.dt * a);
+            kahan_add(u, ku, G.dt * w);
           }
-          gu[idx] = u; gv[idx] = v; gap[idx] = a; gkc[idx] = pack_comp(ku, kv);
+          gu[idx] = u; gw[idx] = w; gap[idx] = a; gkc[idx] = pack_comp(ku, kw);
         }
       }
       // own slot only: every thread's flux stage has read su (barrier above)
@@ -690,9 +687,9 @@ __global__ void k_plain_scatter(int64_t first, int64_t count, int nd, int C, int
   for (int c = 0; c < C; ++c) acc[(int64_t)c * G + g] += aloc[(e * C + c) * nd + p];
 }
 
-__global__ void k_plain_update(int64_t G, int C, int quirks, float dt, float acl, float dsp,
+__global__ void k_plain_update(int64_t G, int C, int quirks, float dt,
                                const float *__restrict__ mi, const float *__restrict__ acc,
-                               float *__restrict__ u, float *__restrict__ v, float *__restrict__ ap,
+                               float *__restrict__ u, float *__restrict__ w, float *__restrict__ ap,
                                uint32_t *__restrict__ kc) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= G) return;
@@ -703,14 +700,13 @@ __global__ void k_plain_update(int64_t G, int C, int quirks, float dt, float acl
     int64_t i = (int64_t)c * G + g;
     float a = m * acc[i];
     const uint32_t k2 = kc[i];
-    float pa = ap[i], vv = v[i], kvv = comp_v(k2), uu = u[i], kuu = comp_u(k2);
+    float ww = w[i], kww = comp_v(k2), uu = u[i], kuu = comp_u(k2);
     int reps = (q3 && c == 0) ? 2 : 1;
-    for (int r = 0; r < reps; ++r) {
-      kahan_add(vv, kvv, acl * (a + pa));
-      kahan_add(uu, kuu, fmaf(dt, vv, dsp * a));
-      pa = a;
+    for (int r = 0; r < reps; ++r) {  // w = v + dt/2 a_: w += dt a, u += dt w
+      kahan_add(ww, kww, dt * a);
+      kahan_add(uu, kuu, dt * ww);
     }
-    u[i] = uu; v[i] = vv; ap[i] = a; kc[i] = pack_comp(kuu, kvv);
+    u[i] = uu; w[i] = ww; ap[i] = a; kc[i] = pack_comp(kuu, kww);
   }
 }
 
@@ -918,10 +914,9 @@ int launch_plain_scatter(int64_t first, int64_t count, int nd, int C, int64_t G,
   return last_launch("plain scatter");
 }
 
-int launch_plain_update(int64_t G, int C, int quirks, float dt, float acl, float dsp,
-                        const float *mi, const float *acc, float *u, float *v, float *ap,
-                        uint32_t *kc, hipStream_t st) {
-  k_plain_update<<<nblocks(G, 256), 256, 0, st>>>(G, C, quirks, dt, acl, dsp, mi, acc, u, v, ap, kc);
+int launch_plain_update(int64_t G, int C, int quirks, float dt, const float *mi, const float *acc,
+                        float *u, float *w, float *ap, uint32_t *kc, hipStream_t st) {
+  k_plain_update<<<nblocks(G, 256), 256, 0, st>>>(G, C, quirks, dt, mi, acc, u, w, ap, kc);
   return last_launch("plain update");
 }
 

@@ -159,8 +159,10 @@ struct GroupDev {
   const float *par_e;     // E x nv x kpar
   const float *geo_p;     // pointwise: E x ngp x nd
   const float *mi;        // G
-  float *u, *v, *ap;
-  uint32_t *kc;           // Kahan compensations of u and v (bf16 pair per word)
+  // Newmark state: u, w = v + dt/2 a_ (what the update reads), a_ (written
+  // for read-out only: v = w - dt/2 a_)
+  float *u, *w, *ap;
+  uint32_t *kc;           // Kahan compensations of u and w (bf16 pair per word)
   float *acc;
   const SrcDev *src;
   const float *stf;
@@ -170,7 +172,7 @@ struct GroupDev {
   int64_t G;
   int64_t nsteps;
   int64_t *step;          // device step counter of this group
-  float dt, acl, dsp;
+  float dt, acl;           // acl = dt/2 (Order2Newmark.cpp:89), host conversions of w
   int32_t quirks;
   int32_t pad;
   // tile kernel (structured meshes): fields in tile-major node numbering
@@ -273,9 +275,9 @@ int launch_plain_stiffness(int dim, int order, int physics, int gmode, int64_t E
                            const PlainArgs &a, hipStream_t st);
 int launch_plain_scatter(int64_t first, int64_t count, int nd, int C, int64_t G,
                          const uint32_t *conn, const float *aloc, float *acc, hipStream_t st);
-int launch_plain_update(int64_t G, int C, int quirks, float dt, float acl, float dsp,
-                        const float *mi, const float *acc, float *u, float *v, float *ap,
-                        uint32_t *kc, hipStream_t st);
+// Newmark update in the (u, w = v + dt/2 a_) form: w += dt a, u += dt w.
+int launch_plain_update(int64_t G, int C, int quirks, float dt, const float *mi, const float *acc,
+                        float *u, float *w, float *ap, uint32_t *kc, hipStream_t st);
 int launch_receivers(const RecArgs &a, int C, hipStream_t st);
 // Advances every group's step counter by k (clamped at its nsteps).
 int launch_advance(int ngroups, const GroupDev *groups, int64_t k, hipStream_t st);

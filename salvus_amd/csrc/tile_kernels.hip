@@ -21,8 +21,10 @@
 //           position containing a node holds the tile's total for it;
 //   finish  every lattice slot is stored / accumulated in HBM, or -- on the
 //           last tile colour touching it -- finished: a = mi (acc + f);
-//           Kahan v += dt/2 (a + a_); Kahan u += dt v + dt^2/2 a; a_ = a
-//           (Order2Newmark.cpp:68-115).
+//           Kahan w += dt a; Kahan u += dt w; a_ = a, where w = v + dt/2 a_
+//           is the state the update reads (Order2Newmark.cpp:68-115:
+//           v += dt/2 (a + a_), u += dt v + dt^2/2 a, a_ = a, rewritten so
+//           that a_ is only written).
 // The operator is the one of TensorQuad.cpp:176-210/:350-397 and
 // Hexahedra.cpp:579-754 (sum-factorised gradient, J^-T contraction,
 // test-and-integrate) with Scalar.cpp:42-71, Elastic2D.cpp:53-82 and
@@ -129,7 +131,7 @@ struct TS {
   // Threads: at least those of a chunk, and enough that every thread's
   // lattice slots fit the registers that hold finish inputs loaded ahead
   // (KFMAX slots), so that no slot waits for its loads in the finish.
-  static constexpr int KFMAX = 64 / (5 * C + 2);
+  static constexpr int KFMAX = 64 / (5 * C + 2);  // registers per slot held ahead: budgeted as before
   static constexpr int BTF = (((NTL + KFMAX - 1) / KFMAX + 63) / 64) * 64;
   static constexpr int BT = BTF > BT0 ? (BTF < 1024 ? BTF : 1024) : BT0;
   static constexpr int NIT = (NTL + BT - 1) / BT;           // lattice slots per thread
@@ -286,7 +288,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
   const int pl = ci + (DIM == 3 ? n * cj : 0);  // element-local index of the column's node 0
 
   const gptr<float> gu = glb(G.u);
-  const gptr<float> gv = glb(G.v);
+  const gptr<float> gw = glb(G.w);
   const gptr<float> gap = glb(G.ap);
   const gptr<uint32_t> gkc = glb(G.kc);
   const gptr<float> gacc = glb(G.acc);
@@ -506,7 +508,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
     // are loaded here so that their latency overlaps the face passes
     struct SlotIn {
       uint32_t w;
-      float mi, prev[C], ap[C], v[C];
+      float mi, prev[C], vw[C];  // vw: w = v + dt/2 a_
       uint32_t kc[C];
     };
     auto load_slot = [&](int r, SlotIn &in) {
@@ -521,8 +523,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
       for (int c = 0; c < C; ++c) {
         const uint32_t idx = (s * C + c) * GG + g;
         in.prev[c] = (valid && !first) ? gacc[idx] : 0.f;
-        in.ap[c] = fin ? gap[idx] : 0.f;
-        in.v[c] = fin ? gv[idx] : 0.f;
+        in.vw[c] = fin ? gw[idx] : 0.f;
         in.kc[c] = fin ? gkc[idx] : 0u;
       }
     };
@@ -595,15 +596,14 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
         }
         if (q3 && c == 2) continue;
         const float a = in.mi * t;
-        float ap = in.ap[c], v = in.v[c], kv = comp_v(in.kc[c]);
+        float w = in.vw[c], kw = comp_v(in.kc[c]);
         float u = su[c * T::NTL + l], ku = comp_u(in.kc[c]);
         const int reps = (q3 && c == 0) ? 2 : 1;
         for (int rep = 0; rep < reps; ++rep) {
-          kahan(v, kv, G.acl * (a + ap));
-          kahan(u, ku, fmaf(G.dt, v, G.dsp * a));
-          ap = a;
+          kahan(w, kw, G.dt * a);
+          kahan(u, ku, G.dt * w);
         }
-        gu[idx] = u; gv[idx] = v; gap[idx] = a; gkc[idx] = pack_comp(ku, kv);
+        gu[idx] = u; gw[idx] = w; gap[idx] = a; gkc[idx] = pack_comp(ku, kw);
       }
     };
 #pragma unroll
