@@ -5,7 +5,9 @@ copy kernels below move exactly `--mib` MiB read and `--mib` MiB written per
 launch; the counters' per-dispatch KiB over those sizes are the factors that
 turn a kernel's counter readings into HBM bytes.
 
-python -m salvus_amd.hbm_calibration [--mib 1024] [--reps 3]
+python hbm_calibration.py [--mib 1024] [--reps 3]   (from the repository root)
+
+Measurement tooling beside bench.py; the library itself never imports torch.
 """
 import argparse
 
