@@ -243,8 +243,14 @@ int sem_server_submit(sem_server *server, sem_mesh *mesh, int64_t nsteps, double
 /* status[0..5] = {active, pending, finished, steps advanced, next arrival
  *                 (-1: none), device bytes held by active requests} */
 int sem_server_pump(sem_server *server, double *status);
-/* info[0..5] = {state (0 pending, 1 active, 2 done, 3 failed), arrival,
- *               admission, completion, steps done, device bytes} */
+/* Cancels a request (no reference counterpart): a pending request leaves
+ * the queue; an active one
+ * is retired at once, its device memory released and its partial results
+ * dropped.  Finished or cancelled requests are an error.  The other requests'
+ * results do not change (each request's arithmetic is its own). */
+int sem_server_cancel(sem_server *server, int request);
+/* info[0..5] = {state (0 pending, 1 active, 2 done, 3 failed, 4 cancelled),
+ *               arrival, admission, completion, steps done, device bytes} */
 int sem_server_request(const sem_server *server, int request, double *info);
 /* seismograms of a finished request: nsteps x receivers x components */
 int sem_server_result(const sem_server *server, int request, float *seis);

@@ -25,7 +25,7 @@ namespace sem {
 struct Request {
   Shot shot;
   double arrival = 0, t_admit = -1, t_done = -1;
-  int state = 0;  // 0 pending, 1 active, 2 done, 3 failed
+  int state = 0;  // 0 pending, 1 active, 2 done, 3 failed, 4 cancelled
   int64_t bytes = 0, steps = 0;
   std::vector<float> seis;
 };
@@ -60,6 +60,22 @@ static int64_t request_bytes(const sem_server *s, const Shot &sh) {
   return b;
 }
 
+// Removes active group gi (its request's device memory and its shot's host
+// data) and renumbers the groups behind it.
+static void drop_active(sem_server *s, int gi) {
+  sem_batch *b = s->b;
+  Group &g = b->groups[gi];
+  for (int k : g.shots) b->shots[k] = Shot{};  // the shot's sources, receivers, fields
+  free_group(g);
+  b->groups.erase(b->groups.begin() + gi);
+  b->h_groups.erase(b->h_groups.begin() + gi);
+  s->active.erase(s->active.begin() + gi);
+  for (int gj = gi; gj < (int)b->groups.size(); ++gj)
+    for (int k : b->groups[gj].shots) b->shots[k].group = gj;
+  // trailing released shots are not referenced any more
+  while (!b->shots.empty() && !b->shots.back().mesh) b->shots.pop_back();
+}
+
 static void retire(sem_server *s, double now) {
   sem_batch *b = s->b;
   for (int gi = (int)s->active.size() - 1; gi >= 0; --gi) {
@@ -75,10 +91,7 @@ static void retire(sem_server *s, double now) {
     r.t_done = now;
     s->used -= r.bytes;
     ++s->done;
-    free_group(g);
-    b->groups.erase(b->groups.begin() + gi);
-    b->h_groups.erase(b->h_groups.begin() + gi);
-    s->active.erase(s->active.begin() + gi);
+    drop_active(s, gi);
   }
 }
 
@@ -198,6 +211,7 @@ int sem_server_pump(sem_server *s, double *status) {
     if (rc) {
       free_group(b->groups[gi]);
       b->groups.pop_back();
+      b->shots.pop_back();  // admitted again later from the request itself
       if ((int)b->h_groups.size() > gi) b->h_groups.resize(gi);
       if (rc == -3) break;  // device memory short: retry after retirements
       return rc;
@@ -236,6 +250,35 @@ int sem_server_pump(sem_server *s, double *status) {
   status[3] = (double)k;
   status[4] = s->pending.empty() ? -1.0 : s->reqs[s->pending.front()].arrival;
   status[5] = (double)s->used;
+  return 0;
+}
+
+int sem_server_cancel(sem_server *s, int request) {
+  if (request < 0 || request >= (int)s->reqs.size()) { set_error("request index"); return -1; }
+  Request &r = s->reqs[request];
+  if (r.state == 0) {
+    for (auto it = s->pending.begin(); it != s->pending.end(); ++it)
+      if (*it == request) {
+        s->pending.erase(it);
+        break;
+      }
+  } else if (r.state == 1) {
+    HIPCHK(hipSetDevice(s->b->device));
+    HIPCHK(hipStreamSynchronize(s->b->stream));
+    for (int gi = 0; gi < (int)s->active.size(); ++gi)
+      if (s->active[gi] == request) {
+        drop_active(s, gi);
+        break;
+      }
+    s->used -= r.bytes;
+    SEM_TRY(finalize_groups(s->b, 0));
+    s->b->prepared = true;
+  } else {
+    set_error("request already finished or cancelled");
+    return -1;
+  }
+  r.state = 4;
+  r.t_done = clock_of(s);
   return 0;
 }
 

@@ -40,6 +40,7 @@ EXPORTS = (
     "sem_server_submit", "sem_server_pump", "sem_server_request", "sem_server_result",
     "sem_mesh_autotune", "sem_shot_add_file_source", "sem_receivers_write_h5",
     "sem_source_read_h5", "sem_movie_write_h5", "sem_mesh_apply_stiffness_noalloc",
+    "sem_server_cancel",
 )
 SOURCE_DOUBLES = 19
 
@@ -104,6 +105,7 @@ def lib():
         L.sem_server_pump.argtypes = [_VP, _D]
         L.sem_server_request.argtypes = [_VP, C.c_int, _D]
         L.sem_server_result.argtypes = [_VP, C.c_int, _F]
+        L.sem_server_cancel.argtypes = [_VP, C.c_int]
         L.sem_mesh_autotune.argtypes = [_VP, C.c_int, C.c_int, _I64]
         L.sem_shot_add_file_source.argtypes = [_VP, C.c_int, _D, C.c_int, C.c_int64, _D]
         L.sem_receivers_write_h5.argtypes = [C.c_char_p, C.c_int, C.c_int,
@@ -499,7 +501,7 @@ class Server:
     order while the HBM budget (bytes, 0 = free memory less a reserve) and
     max_active (0 = no cap) allow."""
 
-    STATES = {0: "pending", 1: "active", 2: "done", 3: "failed"}
+    STATES = {0: "pending", 1: "active", 2: "done", 3: "failed", 4: "cancelled"}
 
     def __init__(self, path=PATH_FUSED, hbm_budget=0, max_active=0, block_steps=32, device=0):
         self._h = _VP()
@@ -546,6 +548,11 @@ class Server:
         _check(lib().sem_server_request(self._h, rid, _d(info)))
         return dict(state=self.STATES[int(info[0])], arrival=info[1], admit=info[2], done=info[3],
                     steps=int(info[4]), bytes=int(info[5]))
+
+    def cancel(self, rid):
+        """A pending request leaves the queue; an active one is retired at
+        once (device memory released, partial results dropped)."""
+        _check(lib().sem_server_cancel(self._h, rid))
 
     def result(self, rid):
         mesh, nsteps, nrec = self.requests[rid]

@@ -126,3 +126,83 @@ def test_served_shot_matches_oracle(case):
     assert pb.rel_l2(got, ref) < TOL_RUN
     plain = _alone(M, P, path=sem.PATH_PLAIN, max_active=1)
     assert pb.rel_l2(plain, ref) < TOL_RUN
+
+
+def test_cancel_pending_and_active():
+    """A cancelled active request leaves at once (its memory released), a
+    cancelled pending one never starts; the others finish with their alone
+    results."""
+    Ps = _problems()
+    Ms = [_mesh(P) for P in Ps]
+    alone = [_alone(M, P) for M, P in zip(Ms, Ps)]
+    S = sem.Server(block_steps=8)
+    ids = [S.submit(M, P["nsteps"], P["dt"], _sources(P), P["recs"]) for M, P in zip(Ms, Ps)]
+    late = S.submit(Ms[0], Ps[0]["nsteps"], Ps[0]["dt"], _sources(Ps[0]), Ps[0]["recs"],
+                    arrival=S.clock() + 3600.0)
+    st = S.pump()
+    assert st["active"] == 4 and st["pending"] == 1
+    held = st["bytes"]
+    S.cancel(ids[1])
+    S.cancel(late)
+    st = S.pump()
+    assert st["active"] == 3 and st["pending"] == 0
+    assert st["bytes"] == held - S.request(ids[1])["bytes"]
+    with pytest.raises(sem.SemError):
+        S.cancel(ids[1])
+    S.drain()
+    assert S.request(ids[1])["state"] == "cancelled" and S.request(late)["state"] == "cancelled"
+    assert S.request(late)["admit"] < 0
+    for k in (0, 2, 3):
+        assert S.request(ids[k])["state"] == "done"
+        np.testing.assert_array_equal(S.result(ids[k]), alone[k])
+
+
+def _full_size_models():
+    """One model per configuration-5 category at the top of its size range:
+    800^2 acoustic quads, 1000^2 elastic quads, 96^3 elastic hexes."""
+    from salvus_amd import serve
+    return [serve.build_model(2, 0, 800, 501), serve.build_model(2, 1, 1000, 502),
+            serve.build_model(3, 1, 96, 503)]
+
+
+def test_full_size_requests_same_alone_and_in_mix():
+    """Configuration 5 at its largest sizes: the three requests, arriving one
+    after another while the earlier ones run, share the device; each result
+    equals the same request served alone, bit for bit."""
+    from salvus_amd import serve
+    models = _full_size_models()
+    rng = np.random.default_rng(5)
+    # sources next to each model's first receiver, so that a few dozen steps
+    # already record the wave
+    reqs = []
+    for k, n in enumerate((96, 80, 64)):
+        m = models[k]
+        pos = np.zeros(3)
+        pos[:m["dim"]] = np.asarray(m["recs"][0]) / m["L"] - 0.4 / m["n"]
+        reqs.append(dict(model=k, arrival=0.0, nsteps=n, pos=pos, mt=rng.normal(0.0, 1.0, (3, 3)),
+                         seed=k))
+    alone = []
+    for r in reqs:
+        m = models[r["model"]]
+        S = sem.Server(block_steps=16)
+        i = S.submit(m["mesh"], r["nsteps"], m["dt"], serve._sources(m, r), m["recs"])
+        S.drain()
+        alone.append(S.result(i))
+        S.close()
+    S = sem.Server(block_steps=16)
+    ids = []
+    for k, r in enumerate(reqs):
+        m = models[r["model"]]
+        ids.append(S.submit(m["mesh"], r["nsteps"], m["dt"], serve._sources(m, r), m["recs"],
+                            arrival=S.clock()))
+        st = S.pump()  # the next request arrives while this one runs
+        assert st["active"] == k + 1
+    S.drain()
+    admits = [S.request(i)["admit"] for i in ids]
+    assert admits[1] < S.request(ids[0])["done"] and admits[2] < S.request(ids[0])["done"]
+    for i, a in zip(ids, alone):
+        assert S.request(i)["state"] == "done"
+        assert np.abs(a).max() > 0
+        np.testing.assert_array_equal(S.result(i), a)
+    for m in models:
+        m["mesh"].close()
