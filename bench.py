@@ -260,15 +260,18 @@ def main():
         "data": "synthetic (seeded models and sources)",
         "config": {"workload": desc, "meshes": [int(M.E) for M in meshes], "shots": len(ids),
                    "dof_updates_per_step": dofs,
-                   "path": "batched (%s)" % ("tile kernel" if all(M.tile_shape for M in meshes)
-                                             else "element kernel" if not any(M.tile_shape for M in meshes)
+                   "path": "batched (%s)" % ("tile kernel" if cfg["tiled_groups"] == cfg["groups"]
+                                             else "element kernel" if cfg["tiled_groups"] == 0
                                              else "tile and element kernels"),
                    "tile_shapes": [list(M.tile_shape) if M.tile_shape else None for M in meshes],
                    "shots_per_group": cfg["shots_per_group"], "groups": cfg["groups"],
                    "graph_block_steps": cfg["block_steps"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_tile<%d,4,%s,%s>" % (dim, ("acoustic", "elastic")[physics], shape),
+                     "kernel": ("k_tile<%d,4,%s,%s>" % (dim, ("acoustic", "elastic")[physics], shape)
+                                if cfg["tiled_groups"] else
+                                "k_fused<%d,4,%s,%s>" % (dim, ("acoustic", "elastic")[physics],
+                                                         "affine" if meshes[-1].affine else "pointwise")),
                      "algorithmic_bytes_per_launch": per_launch_bytes,
                      "bytes_model": "element formulation: per element n^d (4 + 12 c S) + 4 x "
                                     "(affine factors + vertex material); per node S (4 + 36 c)",

@@ -205,8 +205,9 @@ int sem_source_read_h5(const char *path, const char *name, int64_t nsteps, int n
  * appended (else -1 and nothing is written).  Names hold no '/'. */
 int sem_movie_write_h5(const char *path, int append, int nfields, const char *const *names,
                        int64_t n, const double *values, double time);
-/* Launch configuration chosen: cfg[0..3] = {elems/block, shots/group,
- * graph block steps, groups}. */
+/* Launch configuration chosen: cfg[0..4] = {elems/block, shots/group,
+ * graph block steps, groups, groups on the tile kernel (the others take the
+ * element kernel)}. */
 int sem_batch_config(const sem_batch *batch, int64_t *cfg);
 /* Timing of the dominant kernel during the last sem_batch_run when the
  * environment sets SEM_TIME_KERNELS=1 (steps then run without graphs): total

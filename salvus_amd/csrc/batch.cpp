@@ -970,6 +970,9 @@ int sem_batch_config(const sem_batch *b, int64_t *cfg) {
   cfg[1] = b->spg;
   cfg[2] = b->block_steps;
   cfg[3] = (int64_t)b->groups.size();
+  int64_t tiled = 0;
+  for (const auto &g : b->groups) tiled += g.tiled ? 1 : 0;
+  cfg[4] = tiled;
   return 0;
 }
 

@@ -115,6 +115,17 @@ const H5 *hdf5() {
             sym(lib, "H5P_CLS_DATASET_CREATE_ID_g", h.dcpl_class) &&
             sym(lib, "H5T_NATIVE_FLOAT_g", h.native_float) &&
             sym(lib, "H5T_NATIVE_DOUBLE_g", h.native_double);
+  // hid_t is 64-bit from HDF5 1.10 on: check the version before any
+  // hid_t global (H5T_NATIVE_*_g) is read
+  int (*libversion)(unsigned *, unsigned *, unsigned *) = nullptr;
+  unsigned vmaj = 0, vmin = 0, vrel = 0;
+  if (ok && (!sym(lib, "H5get_libversion", libversion) || libversion(&vmaj, &vmin, &vrel) < 0 ||
+             vmaj < 1 || (vmaj == 1 && vmin < 10))) {
+    dlclose(lib);
+    sem::set_error("libhdf5 " + std::to_string(vmaj) + "." + std::to_string(vmin) + "." +
+                   std::to_string(vrel) + " is too old (the HDF5 >= 1.10 API is required)");
+    return nullptr;
+  }
   if (!ok || h.open() < 0) {
     dlclose(lib);
     sem::set_error("libhdf5 found but unusable (needs the HDF5 >= 1.10 API)");

@@ -410,10 +410,10 @@ class Batch:
         write_movie_frame(path, frame, self.progress(shot)[1], append)
 
     def config(self):
-        c = np.zeros(4, dtype=np.int64)
+        c = np.zeros(5, dtype=np.int64)
         _check(lib().sem_batch_config(self._h, _l(c)))
         return dict(elems_per_block=int(c[0]), shots_per_group=int(c[1]),
-                    block_steps=int(c[2]), groups=int(c[3]))
+                    block_steps=int(c[2]), groups=int(c[3]), tiled_groups=int(c[4]))
 
     def kernel_time(self):
         ms, n = np.zeros(1), np.zeros(1, dtype=np.int64)
