@@ -19,15 +19,17 @@ value        DOF-updates/s of the batched path: every shot in one set of tile
              resident in HBM, K timed steps
 plain        the same shots through the plain one-shot-per-launch path
              (gather / stiffness / scatter / update kernels per shot)
-accuracy     receiver relative L2 of both FP32 HIP paths against the FP64
-             oracle on a bounded sample of the workload (same material law,
-             source type and receiver layout on a smaller mesh)
+accuracy     receiver and final-wavefield relative L2 of both FP32 HIP paths
+             against FP64 oracle runs of the workload's full length
+             (committed fixtures: the 32^3 meshes of cfg4 / cfg3, cfg2 at
+             full size)
 roofline     the tile kernel's algorithmic HBM bytes per launch over its
              average duration (HIP events on the batch stream); traffic from
              the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the
              same workload when present
-cpu_baseline the FP64 oracle (a port of the reference's CPU path) on the
-             bounded sample, 1 thread, on this host
+cpu_baseline the FP64 oracle (a port of the reference's CPU path) on the same
+             mesh for a bounded number of steps, 1 thread and every host
+             core, on this host
 serving      (cfg4) a configuration-5 request stream through the
              continuous-batching server and the plain FIFO server
 
@@ -126,25 +128,6 @@ def make_batch(sem, meshes, shots, path, nsteps, spg=0):
         ids.append(k)
     B.prepare(path, 0, spg)
     return B, ids
-
-
-def sample(workloads, name):
-    """Bounded accuracy / CPU-baseline sample of a workload."""
-    if name == "cfg4":
-        W = workloads.cfg4(sizes=(10,), shots_per_mesh=1, nsteps=250)
-        mw = W["meshes"][0]
-        return dict(dim=3, physics=1, cells=mw["cells"], coords=mw["coords"], par=mw["par"],
-                    dt=mw["dt"], nsteps=250, sources=mw["shots"], receivers=mw["receivers"],
-                    desc="cfg4 law on a 10^3 mesh, 1 moment-tensor shot, 250 steps, 3 receivers")
-    if name == "cfg3":
-        W = workloads.cfg3(n=16, nsteps=300, nrec_side=4)
-        desc = "cfg3 law on a 16^3 mesh, 300 steps, 16 receivers"
-    else:
-        W = workloads.cfg2(n=100, nsteps=300)
-        desc = "cfg2 law on 100^2 quads, 300 steps, 100 receivers"
-    return dict(dim=W["dim"], physics=W["physics"], cells=W["cells"], coords=W["coords"],
-                par=W["par"], dt=W["dt"], nsteps=300, sources=W["sources"],
-                receivers=W["receivers"], desc=desc)
 
 
 def cpu_model():
@@ -302,57 +285,67 @@ def main():
     for M in meshes:
         M.close()
 
-    # ---------------- accuracy sample + CPU baseline (FP64 oracle, 1 thread)
+    # ---------------- accuracy at full length + CPU baseline (FP64 oracle)
     if not args.no_cpu:
         from oracle import pyoracle as po
+        from tests.golden.make_fullrun_fixtures import input_digest, problem
         from tests.problems import rel_l2
-        sm = sample(workloads, args.workload)
-        O = po.Sim(sm["dim"], 4, sm["physics"], sm["cells"], sm["coords"], sm["par"])
-        for s in sm["sources"]:
-            if s["kind"] == "moment":
-                O.add_moment_source(s["loc"], s["M"], s["amp"], s["f0"], s["t0"])
-            else:
-                O.add_source(s["loc"], s["amp"], s["f0"], s["t0"], s["dir"])
-        for r in sm["receivers"]:
-            O.add_receiver(r)
-        t0 = time.perf_counter()
-        ref = O.run(sm["dt"], sm["nsteps"])
-        cpu_s = time.perf_counter() - t0
-        ref_u = O.get_field("u")
-        # the same sample on every host core (the oracle's colour-parallel loop)
-        nthr = min(os.cpu_count() or 1, 16)
-        O2 = po.Sim(sm["dim"], 4, sm["physics"], sm["cells"], sm["coords"], sm["par"])
-        for s in sm["sources"]:
-            if s["kind"] == "moment":
-                O2.add_moment_source(s["loc"], s["M"], s["amp"], s["f0"], s["t0"])
-            else:
-                O2.add_source(s["loc"], s["amp"], s["f0"], s["t0"], s["dir"])
-        t0 = time.perf_counter()
-        O2.run(sm["dt"], sm["nsteps"], nthreads=nthr)
-        cpu_all_s = time.perf_counter() - t0
-        Ms = sem.Mesh(sm["dim"], 4, sm["physics"], sm["cells"], sm["coords"], sm["par"])
+        fx_name = {"cfg4": "cfg4-32", "cfg3": "cfg3-32", "cfg2": "cfg2"}[args.workload]
+        F = np.load(ROOT / "tests" / "golden" / ("fullrun_%s.npz" % fx_name))
+        fdim, fphys, fcells, fcoords, fpar, fdt, fsteps, fshots, frecs, _ = problem(fx_name)
+        assert input_digest(fcells, fcoords, fpar) == str(F["digest"]), "fixture inputs differ"
+        Mf = sem.Mesh(fdim, 4, fphys, fcells, fcoords, fpar)
         errs, field_errs = {}, {}
         for name, path in (("batched", sem.PATH_FUSED), ("plain", sem.PATH_PLAIN)):
-            Bs, ks = make_batch(sem, [Ms], [(0, sm["dt"], sm["sources"], sm["receivers"])], path,
-                                sm["nsteps"])
-            Bs.run(sm["nsteps"])
-            errs[name] = rel_l2(Bs.seismograms(ks[0]), ref)
-            field_errs[name] = rel_l2(Bs.get_field(ks[0], "u"), ref_u)
+            Bs, ks = make_batch(sem, [Mf], [(0, fdt, srcs, frecs) for srcs in fshots], path, fsteps)
+            Bs.run(fsteps)
+            errs[name] = max(rel_l2(Bs.seismograms(k), F["seis%d" % i].astype(np.float64))
+                             for i, k in enumerate(ks))
+            field_errs[name] = max(rel_l2(Bs.get_field(k, "u").ravel()[::int(F["stride"])],
+                                          F["u_sample%d" % i]) for i, k in enumerate(ks))
             Bs.close()
-        dof_s = Ms.c * Ms.G * sm["nsteps"]
-        Ms.close()
+        Mf.close()
         out["accuracy"] = {"receiver_rel_l2_batched": errs["batched"],
                            "receiver_rel_l2_plain": errs["plain"],
                            "final_u_rel_l2_batched": field_errs["batched"],
                            "final_u_rel_l2_plain": field_errs["plain"], "bar": 1e-5,
-                           "sample": sm["desc"]}
-        out["cpu_baseline"] = {"value": dof_s / cpu_s, "unit": "DOF-updates/s", "cores": 1,
-                               "kind": "port",
-                               "sample": "FP64 oracle, %s (%.1f s)" % (sm["desc"], cpu_s)}
-        out["cpu_baseline_all_cores"] = {"value": dof_s / cpu_all_s, "unit": "DOF-updates/s",
+                           "sample": "against FP64 oracle runs of the full length (fixture "
+                                     "tests/golden/fullrun_%s.npz): %s, %d steps, %d shot(s), "
+                                     "worst shot; final u on every %d-th DOF"
+                                     % (fx_name, fx_name, fsteps, len(fshots), int(F["stride"]))}
+        log("accuracy vs FP64 over %d steps: receivers %.2e / %.2e (batched / plain)"
+            % (fsteps, errs["batched"], errs["plain"]))
+        # CPU baseline: the oracle on this workload's mesh (the 32^3 sub-box
+        # for the 3-D workloads, the full 500^2 mesh for cfg2), a bounded
+        # number of steps, 1 thread and every host core
+        ksteps = {"cfg4": 15, "cfg3": 40, "cfg2": 50}[args.workload]
+        src0 = fshots[0]
+        times = {}
+        nthr = min(os.cpu_count() or 1, 16)
+        for nt in (1, nthr):
+            O = po.Sim(fdim, 4, fphys, fcells, fcoords, fpar)
+            for sp in src0:
+                if sp["kind"] == "moment":
+                    O.add_moment_source(sp["loc"], sp["M"], sp["amp"], sp["f0"], sp["t0"])
+                else:
+                    O.add_source(sp["loc"], sp["amp"], sp["f0"], sp["t0"], sp["dir"])
+            for r in frecs:
+                O.add_receiver(r)
+            t0 = time.perf_counter()
+            O.run(fdt, ksteps, nthreads=nt)
+            times[nt] = time.perf_counter() - t0
+            dof_s = O.c * O.G * ksteps
+            del O
+        desc = "FP64 oracle, %s mesh (%d elements, %d DOFs), %d steps of shot 0" % (
+            fx_name, len(fcells), dof_s // ksteps, ksteps)
+        out["cpu_baseline"] = {"value": dof_s / times[1], "unit": "DOF-updates/s", "cores": 1,
+                               "kind": "port", "sample": "%s (%.1f s)" % (desc, times[1])}
+        out["cpu_baseline_all_cores"] = {"value": dof_s / times[nthr], "unit": "DOF-updates/s",
                                          "cores": nthr, "kind": "port",
-                                         "sample": "the same, colour-parallel (%.1f s)" % cpu_all_s}
+                                         "sample": "the same, colour-parallel (%.1f s)" % times[nthr]}
         out["cpu_host"] = {"nproc": os.cpu_count(), "model": cpu_model()}
+        log("CPU oracle: %.3g DOF-updates/s on 1 core, %.3g on %d" %
+            (dof_s / times[1], dof_s / times[nthr], nthr))
 
     # ---------------- serving sample: continuous batching vs plain FIFO
     if args.workload == "cfg4" and not args.no_serve:

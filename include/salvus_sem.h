@@ -196,6 +196,17 @@ int sem_receivers_write_h5(const char *path, int append, int nrec, const char *c
  * matrix: component k's series is out[k * nsteps + n]); the dataset must hold
  * exactly nsteps * ncomp values. */
 int sem_source_read_h5(const char *path, const char *name, int64_t nsteps, int ncomp, double *out);
+/* The mesh part of the movie file, written before any frame as the
+ * reference DMViews the mesh into its viewer (Problem.cpp:133-137): creates
+ * (truncates) `path` with /geometry/vertices [num_verts][dim] FP64 and
+ * /topology/cells [num_cells][2^dim] int32 (attribute cell_dim), the names
+ * of PETSc's DMPlex HDF5 view, and /geometry/nodes [num_nodes][dim] FP64,
+ * the coordinates of the global DOFs whose values the frames hold
+ * (sem_mesh_get_node_coords).  Frames are then appended with
+ * sem_movie_write_h5(path, append = 1, ...). */
+int sem_movie_write_mesh_h5(const char *path, int dim, int64_t num_verts, const double *coords,
+                            int64_t num_cells, const int32_t *cells, int64_t num_nodes,
+                            const double *node_xyz);
 /* Problem::saveSolution (Problem.cpp:217-237; the driver calls it after
  * every step, Main.cpp:55-62, with the time after the step): appends one
  * movie frame to `path` (append = 0 creates / truncates it): field k's n

@@ -9,7 +9,14 @@
 //              column-major matrix (component k's series is the k-th run of
 //              nsteps values); fire() returns row `time_idx`
 //              (SourceHdf5.cpp:31-36).
-//   movies     Problem::saveSolution (Problem.cpp:217-237), called after
+//   movies     the mesh first (Problem.cpp:133-137 opens the viewer and
+//              DMViews the mesh before any frame): /geometry/vertices
+//              [V][dim] FP64, /topology/cells [E][2^dim] int32 with the
+//              attribute cell_dim, the names of PETSc's DMPlex HDF5 view;
+//              and /geometry/nodes [G][dim] FP64, the coordinates of the
+//              global DOFs the frames hold (the GLL nodes in DMPlex DOF
+//              order), so that a reader can place each value.  Then
+//              Problem::saveSolution (Problem.cpp:217-237), called after
 //              every step by the driver (Main.cpp:55-62): each named field
 //              of the frame goes to an extendible FP64 dataset
 //              /fields/<name> [frames][n], and the frame's time (the time
@@ -68,7 +75,11 @@ struct H5 {
   herr_t (*sselect_hyperslab)(hid_t, int, const hsize_t *, const hsize_t *, const hsize_t *,
                               const hsize_t *) = nullptr;
   int (*sget_dims)(hid_t, hsize_t *, hsize_t *) = nullptr;
-  const hid_t *native_float = nullptr, *native_double = nullptr, *dcpl_class = nullptr;
+  hid_t (*acreate)(hid_t, const char *, hid_t, hid_t, hid_t, hid_t) = nullptr;
+  herr_t (*awrite)(hid_t, hid_t, const void *) = nullptr;
+  herr_t (*aclose)(hid_t) = nullptr;
+  const hid_t *native_float = nullptr, *native_double = nullptr, *native_int = nullptr,
+              *dcpl_class = nullptr;
 };
 
 template <class F>
@@ -114,7 +125,9 @@ const H5 *hdf5() {
             sym(lib, "H5Sget_simple_extent_dims", h.sget_dims) &&
             sym(lib, "H5P_CLS_DATASET_CREATE_ID_g", h.dcpl_class) &&
             sym(lib, "H5T_NATIVE_FLOAT_g", h.native_float) &&
-            sym(lib, "H5T_NATIVE_DOUBLE_g", h.native_double);
+            sym(lib, "H5T_NATIVE_DOUBLE_g", h.native_double) &&
+            sym(lib, "H5T_NATIVE_INT_g", h.native_int) && sym(lib, "H5Acreate2", h.acreate) &&
+            sym(lib, "H5Awrite", h.awrite) && sym(lib, "H5Aclose", h.aclose);
   // hid_t is 64-bit from HDF5 1.10 on: check the version before any
   // hid_t global (H5T_NATIVE_*_g) is read
   int (*libversion)(unsigned *, unsigned *, unsigned *) = nullptr;
@@ -190,6 +203,18 @@ bool append_row(const H5 *h, hid_t f, const std::string &name, int64_t width, co
   if (ms >= 0) h->sclose(ms);
   if (fs >= 0) h->sclose(fs);
   h->dclose(ds);
+  return ok;
+}
+
+// Writes a fixed 2-D dataset [rows][cols] of `type` at `name`.
+bool write_2d(const H5 *h, hid_t f, const char *name, hid_t type, int64_t rows, int64_t cols,
+              const void *data) {
+  const hsize_t dims[2] = {(hsize_t)rows, (hsize_t)cols};
+  const hid_t sp = h->screate_simple(2, dims, nullptr);
+  const hid_t ds = sp < 0 ? -1 : h->dcreate(f, name, type, sp, kDefault, kDefault, kDefault);
+  const bool ok = ds >= 0 && h->dwrite(ds, type, kDefault, kDefault, kDefault, data) >= 0;
+  if (ds >= 0) h->dclose(ds);
+  if (sp >= 0) h->sclose(sp);
   return ok;
 }
 
@@ -272,6 +297,50 @@ int sem_source_read_h5(const char *path, const char *name, int64_t nsteps, int n
     h->dclose(ds);
   }
   h->fclose(f);
+  return rc;
+}
+
+int sem_movie_write_mesh_h5(const char *path, int dim, int64_t num_verts, const double *coords,
+                            int64_t num_cells, const int32_t *cells, int64_t num_nodes,
+                            const double *node_xyz) {
+  if (!path || (dim != 2 && dim != 3) || num_verts < 1 || !coords || num_cells < 1 || !cells ||
+      num_nodes < 1 || !node_xyz) {
+    sem::set_error("movie_write_mesh_h5: bad arguments");
+    return -1;
+  }
+  const int nv = dim == 2 ? 4 : 8;
+  for (int64_t k = 0; k < num_cells * nv; ++k)
+    if (cells[k] < 0 || cells[k] >= num_verts) {
+      sem::set_error("movie_write_mesh_h5: cell vertex out of range");
+      return -1;
+    }
+  const H5 *h = hdf5();
+  if (!h) return -2;
+  const hid_t f = h->fcreate(path, kAccTrunc, kDefault, kDefault);
+  if (f < 0) { sem::set_error(std::string("cannot create ") + path); return -2; }
+  int rc = 0;
+  for (const char *g : {"geometry", "topology"}) {
+    const hid_t gid = h->gcreate(f, g, kDefault, kDefault, kDefault);
+    if (gid < 0) rc = -2;
+    else h->gclose(gid);
+  }
+  if (!rc && !(write_2d(h, f, "geometry/vertices", *h->native_double, num_verts, dim, coords) &&
+               write_2d(h, f, "topology/cells", *h->native_int, num_cells, nv, cells) &&
+               write_2d(h, f, "geometry/nodes", *h->native_double, num_nodes, dim, node_xyz)))
+    rc = -2;
+  if (!rc) {  // cell_dim on /topology/cells
+    const hid_t ds = h->dopen(f, "topology/cells", kDefault);
+    const hsize_t one = 1;
+    const hid_t sp = h->screate_simple(1, &one, nullptr);
+    const hid_t at = (ds < 0 || sp < 0) ? -1 : h->acreate(ds, "cell_dim", *h->native_int, sp, kDefault, kDefault);
+    const int32_t cd = dim;
+    if (at < 0 || h->awrite(at, *h->native_int, &cd) < 0) rc = -2;
+    if (at >= 0) h->aclose(at);
+    if (sp >= 0) h->sclose(sp);
+    if (ds >= 0) h->dclose(ds);
+  }
+  if (rc) sem::set_error(std::string("writing the mesh to ") + path + " failed");
+  if (h->fclose(f) < 0 && !rc) { sem::set_error("H5Fclose failed"); rc = -2; }
   return rc;
 }
 

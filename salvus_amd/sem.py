@@ -40,7 +40,7 @@ EXPORTS = (
     "sem_server_submit", "sem_server_pump", "sem_server_request", "sem_server_result",
     "sem_mesh_autotune", "sem_shot_add_file_source", "sem_receivers_write_h5",
     "sem_source_read_h5", "sem_movie_write_h5", "sem_mesh_apply_stiffness_noalloc",
-    "sem_server_cancel",
+    "sem_server_cancel", "sem_movie_write_mesh_h5",
 )
 SOURCE_DOUBLES = 19
 
@@ -106,6 +106,8 @@ def lib():
         L.sem_server_request.argtypes = [_VP, C.c_int, _D]
         L.sem_server_result.argtypes = [_VP, C.c_int, _F]
         L.sem_server_cancel.argtypes = [_VP, C.c_int]
+        L.sem_movie_write_mesh_h5.argtypes = [C.c_char_p, C.c_int, C.c_int64, _D, C.c_int64, _I32,
+                                              C.c_int64, _D]
         L.sem_mesh_autotune.argtypes = [_VP, C.c_int, C.c_int, _I64]
         L.sem_shot_add_file_source.argtypes = [_VP, C.c_int, _D, C.c_int, C.c_int64, _D]
         L.sem_receivers_write_h5.argtypes = [C.c_char_p, C.c_int, C.c_int,
@@ -203,6 +205,18 @@ def write_movie_frame(path, fields, time, append=True):
                                     vals.shape[1], _d(vals), float(time)))
 
 
+def write_movie_mesh(path, dim, cells, coords, node_xyz):
+    """The movie file's mesh part (written before the frames, as the
+    reference DMViews the mesh first, Problem.cpp:133-137): vertices, cells
+    and the coordinates of the global DOFs the frames hold."""
+    cells = np.ascontiguousarray(cells, dtype=np.int32)
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    xyz = np.ascontiguousarray(node_xyz, dtype=np.float64)
+    nv = 4 if dim == 2 else 8
+    _check(lib().sem_movie_write_mesh_h5(str(path).encode(), dim, coords.size // dim, _d(coords),
+                                         cells.size // nv, _i(cells), xyz.size // dim, _d(xyz)))
+
+
 def topology(dim, order, cells, nvert):
     """Host-only gather map and colouring: (conn[E, nd], colour[E], info)."""
     cells = np.ascontiguousarray(cells, dtype=np.int32)
@@ -228,6 +242,7 @@ class Mesh:
         par = np.ascontiguousarray(vertex_par, dtype=np.float64)
         nv = 4 if dim == 2 else 8
         self.dim, self.order, self.physics, self.quirks = dim, order, physics, quirks
+        self.cells, self.coords = cells, coords  # kept for the movie file's mesh
         self.E = cells.size // nv
         self.V = coords.size // dim
         self._h = _VP()
@@ -407,7 +422,9 @@ class Batch:
             if which not in got:
                 got[which] = self.get_field(shot, [w for w, i in FIELD.items() if i == which][0])
             frame[f] = got[which][k]
-        write_movie_frame(path, frame, self.progress(shot)[1], append)
+        if not append:  # a new movie file starts with its mesh
+            write_movie_mesh(path, m.dim, m.cells, m.coords, m.node_coords())
+        write_movie_frame(path, frame, self.progress(shot)[1], True)
 
     def config(self):
         c = np.zeros(5, dtype=np.int64)

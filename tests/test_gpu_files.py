@@ -115,6 +115,12 @@ def test_movie_every_step(tmp_path):
     assert np.abs(np.stack(want["ux"])).max() > 0
     t = read_dataset(path, "/time", np.float64)[0]
     np.testing.assert_allclose(t, P["dt"] * np.arange(1, 5), rtol=1e-12)
+    # the mesh part precedes the frames: vertices, cells and the DOF coordinates
+    np.testing.assert_array_equal(read_dataset(path, "/geometry/vertices", np.float64)[0],
+                                  P["coords"])
+    np.testing.assert_array_equal(read_dataset(path, "/topology/cells", np.int32)[0], P["cells"])
+    np.testing.assert_array_equal(read_dataset(path, "/geometry/nodes", np.float64)[0],
+                                  M.node_coords())
     with pytest.raises(KeyError, match="not registered"):
         prob.saveSolution(path, ["uz"])
 

@@ -60,6 +60,11 @@ def _libhdf5():
         h.H5Eset_auto2(DEFAULT, None, None)
         h.float_t = HID.in_dll(h, "H5T_NATIVE_FLOAT_g")
         h.double_t = HID.in_dll(h, "H5T_NATIVE_DOUBLE_g")
+        h.int_t = HID.in_dll(h, "H5T_NATIVE_INT_g")
+        h.H5Aopen.restype = HID
+        h.H5Aopen.argtypes = [HID, C.c_char_p, HID]
+        h.H5Aread.argtypes = [HID, HID, C.c_void_p]
+        h.H5Aclose.argtypes = [HID]
         return h
     return None
 
@@ -80,7 +85,7 @@ def read_dataset(path, name, dtype=np.float32):
     ty = H5.H5Dget_type(d)
     size, cls = H5.H5Tget_size(ty), H5.H5Tget_class(ty)
     out = np.zeros([dims[i] for i in range(nd)], dtype=dtype)
-    mem_t = H5.double_t if dtype == np.float64 else H5.float_t
+    mem_t = {np.float64: H5.double_t, np.int32: H5.int_t}.get(dtype, H5.float_t)
     assert H5.H5Dread(d, mem_t, DEFAULT, DEFAULT, DEFAULT, out.ctypes.data) >= 0
     H5.H5Tclose(ty)
     H5.H5Sclose(sp)
@@ -192,3 +197,38 @@ def test_movie_field_names():
     names = sem.movie_field_names(3)
     assert set(names) == {p + s + q for p in "uva" for s in "xyz" for q in ("",)} | {"ax_", "ay_", "az_"}
     assert names["vy"] == (sem.FIELD["v"], 1) and names["az_"] == names["az"] == (sem.FIELD["a_"], 2)
+
+
+def test_movie_mesh_then_frames(tmp_path):
+    """The mesh goes into the movie file before any frame, as the reference
+    DMViews it into the viewer (Problem.cpp:133-137): vertices, cells
+    (cell_dim attribute) under PETSc's DMPlex view names, and the DOF
+    coordinates that place the frames' values; frames then append."""
+    from oracle import pyoracle as po
+    from salvus_amd import meshgen
+    cells, coords = meshgen.box_quad(3, 2, 30.0, 20.0)
+    S = po.Sim(2, 4, 0, cells, coords, np.full(cells.size, 3000.0))
+    xyz = S.node_coords()
+    path = tmp_path / "movie.h5"
+    sem.write_movie_mesh(path, 2, cells, coords, xyz)
+    for k in range(2):
+        sem.write_movie_frame(path, {"u": np.full(S.G, float(k))}, 0.1 * (k + 1), append=True)
+    v, size, cls = read_dataset(path, "/geometry/vertices", np.float64)
+    np.testing.assert_array_equal(v, coords)
+    c, size, cls = read_dataset(path, "/topology/cells", np.int32)
+    assert size == 4 and cls == 0  # H5T_INTEGER, 4 bytes
+    np.testing.assert_array_equal(c, cells)
+    n, _, _ = read_dataset(path, "/geometry/nodes", np.float64)
+    np.testing.assert_array_equal(n, xyz)
+    u, _, _ = read_dataset(path, "/fields/u", np.float64)
+    assert u.shape == (2, S.G) and u[1, 0] == 1.0
+    f = H5.H5Fopen(str(path).encode(), 0, DEFAULT)
+    d = H5.H5Dopen2(f, b"/topology/cells", DEFAULT)
+    a = H5.H5Aopen(d, b"cell_dim", DEFAULT)
+    cd = np.zeros(1, dtype=np.int32)
+    assert a >= 0 and H5.H5Aread(a, H5.int_t, cd.ctypes.data) >= 0 and cd[0] == 2
+    H5.H5Aclose(a)
+    H5.H5Dclose(d)
+    H5.H5Fclose(f)
+    with pytest.raises(sem.SemError, match="out of range"):
+        sem.write_movie_mesh(path, 2, cells + 100, coords, xyz)
