@@ -223,6 +223,9 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
   // and the lattice of shot 0 (every later shot's lattice is loaded during
   // the previous shot's face passes).  Outside the column phase each thread
   // writes and reads only its own slots l = tid + r BT of su.
+  // inverse mass of the slots whose finish inputs are loaded ahead, read
+  // once per tile for all S shots
+  float mik[T::KF];
   {
     const gptr<const uint32_t> words = glb(G.tile_words) + ti.node_off;
     uint32_t wv[T::NIT];
@@ -232,6 +235,11 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
       wv[r] = l < T::NTL ? words[l] : kSlotEmpty;
     }
     float uv[T::NIT][C];
+#pragma unroll
+    for (int r = 0; r < T::KF; ++r) {
+      const bool fin0 = wv[r] != kSlotEmpty && (wv[r] & kFlagLast) && !A.ku_only;
+      mik[r] = fin0 ? glb(G.mi)[wv[r] & kNodeMask] : 0.f;
+    }
 #pragma unroll
     for (int r = 0; r < T::NIT; ++r) {
       const bool ok = wv[r] != kSlotEmpty;
@@ -511,14 +519,14 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
       float mi, prev[C], vw[C];  // vw: w = v + dt/2 a_
       uint32_t kc[C];
     };
-    auto load_slot = [&](int r, SlotIn &in) {
+    auto load_slot = [&](int r, SlotIn &in, bool keep) {
       const int l = tid + r * T::BT;
       in.w = (r < T::NIT && l < T::NTL) ? sw[l] : kSlotEmpty;
       const bool valid = in.w != kSlotEmpty;
       const uint32_t g = valid ? (in.w & kNodeMask) : 0u;
       const bool first = (in.w & kFlagFirst) != 0;
       const bool fin = valid && (in.w & kFlagLast) && !A.ku_only;
-      in.mi = fin ? gmi[g] : 0.f;
+      in.mi = keep ? 0.f : (fin ? gmi[g] : 0.f);
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         const uint32_t idx = (s * C + c) * GG + g;
@@ -530,7 +538,10 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
     constexpr int KF = T::KF;
     SlotIn pre[KF];
 #pragma unroll
-    for (int r = 0; r < KF; ++r) load_slot(r, pre[r]);
+    for (int r = 0; r < KF; ++r) {
+      load_slot(r, pre[r], true);
+      pre[r].mi = mik[r];
+    }
     const bool more = s + 1 < Sg;
     float nu[T::NIT][C];
 #pragma unroll
@@ -613,7 +624,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
     for (int r0 = KF; r0 < T::NIT; r0 += KB) {
       SlotIn in[KB];
 #pragma unroll
-      for (int r = 0; r < KB; ++r) load_slot(r0 + r, in[r]);
+      for (int r = 0; r < KB; ++r) load_slot(r0 + r, in[r], false);
 #pragma unroll
       for (int r = 0; r < KB; ++r) finish_slot(r0 + r, in[r]);
     }
