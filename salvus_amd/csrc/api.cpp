@@ -154,7 +154,11 @@ static bool pick_tile(int dim, int order, int physics, int *T) {
 static int upload_tile_words(sem_mesh *m) {
   const TileSet &ts = m->tiles;
   if (!m->d_tile_words) SEM_TRY(dalloc(&m->d_tile_words, ts.words.size(), &m->tile_bytes));
+  if (!m->d_tile_slot) SEM_TRY(dalloc(&m->d_tile_slot, ts.slot_of.size(), &m->tile_bytes));
+  if (!m->d_tile_words_lat) SEM_TRY(dalloc(&m->d_tile_words_lat, ts.words_lat.size(), &m->tile_bytes));
+  HIPCHK(hipMemcpyAsync(m->d_tile_words_lat, ts.words_lat.data(), ts.words_lat.size() * 4, hipMemcpyHostToDevice, m->stream));
   HIPCHK(hipMemcpyAsync(m->d_tile_words, ts.words.data(), ts.words.size() * 4, hipMemcpyHostToDevice, m->stream));
+  HIPCHK(hipMemcpyAsync(m->d_tile_slot, ts.slot_of.data(), ts.slot_of.size() * 2, hipMemcpyHostToDevice, m->stream));
   HIPCHK(hipStreamSynchronize(m->stream));
   return 0;
 }
@@ -162,6 +166,8 @@ static int upload_tile_words(sem_mesh *m) {
 void sem::free_tiles(sem_mesh *m) {
   dfree(m->d_tile_info);
   dfree(m->d_tile_words);
+  dfree(m->d_tile_slot);
+  dfree(m->d_tile_words_lat);
   dfree(m->d_telem);
   dfree(m->d_mi_int);
   m->bytes -= std::min(m->bytes, m->tile_bytes);
@@ -262,6 +268,8 @@ static void mesh_free(sem_mesh *m) {
   dfree(m->d_vpar);
   dfree(m->d_tile_info);
   dfree(m->d_tile_words);
+  dfree(m->d_tile_slot);
+  dfree(m->d_tile_words_lat);
   dfree(m->d_telem);
   dfree(m->d_mi_int);
   if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -517,11 +525,12 @@ int sem_mesh_set_dirichlet(sem_mesh *m, const uint8_t *mask) {
     TileSet &ts = m->tiles;
     std::vector<uint8_t> dint(m->G, 0);
     for (int64_t g = 0; g < m->G; ++g) dint[ts.to_internal[g]] = m->dirichlet[g];
-    for (auto &w : ts.words) {
-      if (w == kSlotEmpty) continue;
-      uint32_t node = w & kNodeMask;
-      w = node | (w & flags) | (dint[node] ? kFlagDirichlet : 0u);
-    }
+    for (auto *ws : {&ts.words, &ts.words_lat})
+      for (auto &w : *ws) {
+        if (w == kSlotEmpty) continue;
+        uint32_t node = w & kNodeMask;
+        w = node | (w & flags) | (dint[node] ? kFlagDirichlet : 0u);
+      }
     SEM_TRY(upload_tile_words(m));
   }
   return 0;

@@ -212,6 +212,8 @@ int setup_group(sem_batch *b, int gi) {
   if (g.tiled) {
     h.tiles = m->d_tile_info;
     h.tile_words = m->d_tile_words;
+    h.tile_slot = m->d_tile_slot;
+    h.tile_words_lat = m->d_tile_words_lat;
     h.telem = m->d_telem;
   }
   h.geo_e = m->d_geo_e;

@@ -177,7 +177,9 @@ struct GroupDev {
   int32_t pad;
   // tile kernel (structured meshes): fields in tile-major node numbering
   const TileInfo *tiles;
-  const uint32_t *tile_words;
+  const uint32_t *tile_words;   // per tile, in node order (padding last)
+  const uint16_t *tile_slot;    // lattice slot of each tile word
+  const uint32_t *tile_words_lat;  // per tile, in lattice-slot order
   const float *telem;     // per element in tile order: affine factors, then vertex material
 };
 
@@ -200,7 +202,9 @@ struct TileSet {
   int64_t slots_per_tile = 0;        // (Tx N + 1)(Ty N + 1)(Tz N + 1)
   int elems_per_tile = 0;            // Tx Ty Tz
   std::vector<TileInfo> info;
-  std::vector<uint32_t> words;       // slot -> internal node | flags, or kSlotEmpty
+  std::vector<uint32_t> words;       // per tile, in node order: internal node | flags, or kSlotEmpty
+  std::vector<uint16_t> slot_of;     // lattice slot of each word
+  std::vector<uint32_t> words_lat;   // the same words in lattice-slot order
   std::vector<int32_t> to_internal;  // DMPlex DOF -> internal node
   std::vector<int32_t> elem_order;   // tile-order position -> element (-1: padding)
   std::vector<int32_t> elem_pos;     // element -> tile-order position
@@ -327,6 +331,8 @@ struct sem_mesh {
   sem::TileSet tiles;
   sem::TileInfo *d_tile_info = nullptr;
   uint32_t *d_tile_words = nullptr;
+  uint16_t *d_tile_slot = nullptr;
+  uint32_t *d_tile_words_lat = nullptr;
   float *d_telem = nullptr, *d_mi_int = nullptr;
   size_t bytes = 0;       // device bytes held (tile path included)
   size_t tile_bytes = 0;  // of which the tile path

@@ -162,7 +162,15 @@ struct TS {
   static constexpr int KF = NIT < KFR ? NIT : KFR;
   static constexpr int O_SS = O_SE + E * NED;
   static constexpr int O_SD = O_SS + 4 * kTileMaxSrc + 1;
-  static constexpr int LDS = O_SD + n * n;
+  // Threads take the tile words in node order (coalesced HBM accesses),
+  // except in the high-occupancy variant, whose 80-register budget and
+  // short finish favour lattice order (thread slot = word index, no slot
+  // table).  The small 3-D elastic tiles, whose register budget the slots
+  // would exceed, keep each word's lattice slot in LDS (16 bits per word).
+  static constexpr bool NODE_ORDER = !HO;
+  static constexpr bool SLOT_LDS = NODE_ORDER && C == 3 && E <= 8;
+  static constexpr int O_SL = O_SD + n * n;
+  static constexpr int LDS = O_SL + (SLOT_LDS ? (NTL + 1) / 2 : 0);
 };
 
 __device__ __forceinline__ void kahan(float &s, float &c, float x) {
@@ -221,22 +229,48 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
 
   // ---- once per tile: node words, element data, sources inside the tile,
   // and the lattice of shot 0 (every later shot's lattice is loaded during
-  // the previous shot's face passes).  Outside the column phase each thread
-  // writes and reads only its own slots l = tid + r BT of su.
+  // the previous shot's face passes).  Thread tid owns the tile words
+  // k = tid + r BT, which come in node order, so a wave's global accesses
+  // are coalesced; sl(r) is word k's lattice slot.  Outside the column phase
+  // each thread writes and reads only its own slots of su.
   // inverse mass of the slots whose finish inputs are loaded ahead, read
-  // once per tile for all S shots
-  float mik[T::KF];
+  // once per tile for all S shots (not in the 80-register high-occupancy
+  // variant, which reads it with the other finish inputs)
+  constexpr bool kKeepMi = !HO;
+  float mik[kKeepMi ? T::KF : 1];
+  // lattice slot of the thread's word r: held in registers, in LDS, or
+  // (lattice order) the word index itself
+  constexpr bool kSlotRegs = T::NODE_ORDER && !T::SLOT_LDS;
+  int slr[kSlotRegs ? T::NIT : 1];
+  uint16_t *ssl = reinterpret_cast<uint16_t *>(lds + T::O_SL);
+  const gptr<const uint16_t> slots = glb(G.tile_slot) + ti.node_off;
+  auto sl = [&](int r) {
+    const int k = tid + r * T::BT;
+    if constexpr (!T::NODE_ORDER) {
+      return k < T::NTL ? k : -1;
+    } else if constexpr (kSlotRegs) {
+      return slr[r];
+    } else {
+      return k < T::NTL ? (int)ssl[k] : -1;  // written by this thread at staging
+    }
+  };
   {
-    const gptr<const uint32_t> words = glb(G.tile_words) + ti.node_off;
+    const gptr<const uint32_t> words =
+        (T::NODE_ORDER ? glb(G.tile_words) : glb(G.tile_words_lat)) + ti.node_off;
     uint32_t wv[T::NIT];
 #pragma unroll
     for (int r = 0; r < T::NIT; ++r) {
-      const int l = tid + r * T::BT;
-      wv[r] = l < T::NTL ? words[l] : kSlotEmpty;
+      const int k = tid + r * T::BT;
+      wv[r] = k < T::NTL ? words[k] : kSlotEmpty;
+      if constexpr (T::NODE_ORDER) {
+        const int slot = k < T::NTL ? (int)slots[k] : -1;
+        if constexpr (kSlotRegs) slr[r] = slot;
+        else if (k < T::NTL) ssl[k] = (uint16_t)slot;
+      }
     }
     float uv[T::NIT][C];
 #pragma unroll
-    for (int r = 0; r < T::KF; ++r) {
+    for (int r = 0; r < (kKeepMi ? T::KF : 0); ++r) {
       const bool fin0 = wv[r] != kSlotEmpty && (wv[r] & kFlagLast) && !A.ku_only;
       mik[r] = fin0 ? glb(G.mi)[wv[r] & kNodeMask] : 0.f;
     }
@@ -276,8 +310,8 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
     }
 #pragma unroll
     for (int r = 0; r < T::NIT; ++r) {
-      const int l = tid + r * T::BT;
-      if (l < T::NTL) {
+      const int l = sl(r);
+      if (l >= 0) {
         sw[l] = wv[r];
 #pragma unroll
         for (int c = 0; c < C; ++c) su[c * T::NTL + l] = uv[r][c];
@@ -519,9 +553,8 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
       float mi, prev[C], vw[C];  // vw: w = v + dt/2 a_
       uint32_t kc[C];
     };
-    auto load_slot = [&](int r, SlotIn &in, bool keep) {
-      const int l = tid + r * T::BT;
-      in.w = (r < T::NIT && l < T::NTL) ? sw[l] : kSlotEmpty;
+    auto load_slot = [&](int l, SlotIn &in, bool keep) {
+      in.w = l >= 0 ? sw[l] : kSlotEmpty;
       const bool valid = in.w != kSlotEmpty;
       const uint32_t g = valid ? (in.w & kNodeMask) : 0u;
       const bool first = (in.w & kFlagFirst) != 0;
@@ -539,15 +572,15 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
     SlotIn pre[KF];
 #pragma unroll
     for (int r = 0; r < KF; ++r) {
-      load_slot(r, pre[r], true);
-      pre[r].mi = mik[r];
+      load_slot(sl(r), pre[r], kKeepMi);
+      if constexpr (kKeepMi) pre[r].mi = mik[r];
     }
     const bool more = s + 1 < Sg;
     float nu[T::NIT][C];
 #pragma unroll
     for (int r = 0; r < T::NIT; ++r) {
-      const int l = tid + r * T::BT;
-      const uint32_t w = (more && l < T::NTL) ? sw[l] : kSlotEmpty;
+      const int l = sl(r);
+      const uint32_t w = (more && l >= 0) ? sw[l] : kSlotEmpty;
       const uint32_t g = w != kSlotEmpty ? (w & kNodeMask) : 0u;
 #pragma unroll
       for (int c = 0; c < C; ++c) nu[r][c] = w != kSlotEmpty ? gu[((s + 1) * C + c) * GG + g] : 0.f;
@@ -585,9 +618,8 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
 
     // ---- finish every lattice slot: total of the owner element's face-summed
     // result, accumulation over colours, Newmark update at the last colour
-    auto finish_slot = [&](int r, const SlotIn &in) {
+    auto finish_slot = [&](int l, const SlotIn &in) {
       if (in.w == kSlotEmpty) return;
-      const int l = tid + r * T::BT;
       const int x = l % T::LX, y = (l / T::LX) % T::LY, z = DIM == 3 ? l / T::LXY : 0;
       const int ea = min(x / N, ti.ex - 1), eb = min(y / N, ti.ey - 1);
       const int ez = DIM == 3 ? min(z / N, ti.ez - 1) : 0;
@@ -618,22 +650,29 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
       }
     };
 #pragma unroll
-    for (int r = 0; r < KF; ++r) finish_slot(r, pre[r]);
+    for (int r = 0; r < KF; ++r) finish_slot(sl(r), pre[r]);
     constexpr int KB = C == 1 ? 4 : 2;
 #pragma unroll 1
     for (int r0 = KF; r0 < T::NIT; r0 += KB) {
       SlotIn in[KB];
+      int ls[KB];  // slots of the words k = tid + (r0 + r) BT (runtime r0: read again)
 #pragma unroll
-      for (int r = 0; r < KB; ++r) load_slot(r0 + r, in[r], false);
+      for (int r = 0; r < KB; ++r) {
+        const int k = tid + (r0 + r) * T::BT;
+        ls[r] = (r0 + r < T::NIT && k < T::NTL)
+                    ? (!T::NODE_ORDER ? k : T::SLOT_LDS ? (int)ssl[k] : (int)slots[k])
+                    : -1;
+        load_slot(ls[r], in[r], false);
+      }
 #pragma unroll
-      for (int r = 0; r < KB; ++r) finish_slot(r0 + r, in[r]);
+      for (int r = 0; r < KB; ++r) finish_slot(ls[r], in[r]);
     }
     if (!more) break;
     // own slots only: u of this shot has been read above
 #pragma unroll
     for (int r = 0; r < T::NIT; ++r) {
-      const int l = tid + r * T::BT;
-      if (l < T::NTL) {
+      const int l = sl(r);
+      if (l >= 0) {
 #pragma unroll
         for (int c = 0; c < C; ++c) su[c * T::NTL + l] = nu[r][c];
       }
@@ -803,7 +842,9 @@ static int go_tile(int grid, const FusedArgs &a, hipStream_t st) {
                       : force && !strcmp(force, "normal") ? false
                                                           : 6 * hi < 5 * lo;  // 1.2 hi < lo
     if (high) {
-      hipLaunchKernelGGL((k_tile<D, N, P, X, Y, Z, true>), dim3(grid), dim3(S::BT), shm, st, a);
+      using SH = TS<D, N, P, X, Y, Z, true>;
+      hipLaunchKernelGGL((k_tile<D, N, P, X, Y, Z, true>), dim3(grid), dim3(SH::BT),
+                         sizeof(float) * (size_t)SH::LDS, st, a);
       return 0;
     }
   }

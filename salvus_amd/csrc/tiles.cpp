@@ -10,7 +10,9 @@
 // element's reference axes follow the lattice axes.
 //
 // Tiles are coloured by lattice parity (2^dim colours; tiles of one colour
-// share no node).  Nodes are renumbered tile-major by the tile that finishes
+// share no node).  The words of a tile (node | flags) are stored in the
+// order of their nodes, each with its lattice slot (slot_of), and in
+// lattice order (words_lat, for the high-occupancy acoustic variant).  Nodes are renumbered tile-major by the tile that finishes
 // them (the last colour holding the node): each tile's nodes in slot order,
 // its inside first and then one run per lower/upper face, edge and corner, so
 // that a tile's Newmark state and each piece its neighbours share are
@@ -143,17 +145,35 @@ int build_tiles(sem_mesh *m, int Tx, int Ty, int Tz, TileSet &ts) {
     for (int64_t i = 0; i < cnt; ++i) ts.to_internal[slot_node[t * NTL + order[i]]] = (int32_t)next++;
   }
   if (next != m->G) return 1;
+  // ---- per tile, the slots in the order of their internal node (padding
+  // last): the kernel's threads take the slots in this order, so the lanes
+  // of a wave touch consecutive nodes (one run of the tile's own nodes, then
+  // the runs shared with its neighbours) and every HBM access of the gather
+  // and the finish is coalesced; slot_of gives each position's lattice slot.
+  if (NTL > 65535) return 1;
   ts.words.assign(ntiles * NTL, kSlotEmpty);
-  for (int64_t t = 0; t < ntiles; ++t)
-    for (int64_t s = t * NTL; s < (t + 1) * NTL; ++s) {
-      const int32_t g = slot_node[s];
+  ts.words_lat.assign(ntiles * NTL, kSlotEmpty);
+  ts.slot_of.assign(ntiles * NTL, 0);
+  std::vector<std::pair<int64_t, int32_t>> key(NTL);
+  for (int64_t t = 0; t < ntiles; ++t) {
+    for (int64_t s = 0; s < NTL; ++s) {
+      const int32_t g = slot_node[t * NTL + s];
+      key[s] = {g < 0 ? (int64_t)1 << 40 : (int64_t)ts.to_internal[g], (int32_t)s};
+    }
+    std::sort(key.begin(), key.end());
+    for (int64_t k = 0; k < NTL; ++k) {
+      const int32_t s = key[k].second;
+      ts.slot_of[t * NTL + k] = (uint16_t)s;
+      const int32_t g = slot_node[t * NTL + s];
       if (g < 0) continue;
       uint32_t w = (uint32_t)ts.to_internal[g];
       if (cmin[g] == ts.info[t].colour) w |= kFlagFirst;
       if (cmax[g] == ts.info[t].colour) w |= kFlagLast;
       if (!m->dirichlet.empty() && m->dirichlet[g]) w |= kFlagDirichlet;
-      ts.words[s] = w;
+      ts.words[t * NTL + k] = w;
+      ts.words_lat[t * NTL + s] = w;
     }
+  }
   // ---- colour order of tiles
   for (int64_t t = 0; t < ntiles; ++t) ++ts.colour_start[ts.info[t].colour + 1];
   for (int c = 0; c < (1 << dim); ++c) ts.colour_start[c + 1] += ts.colour_start[c];
