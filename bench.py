@@ -70,15 +70,15 @@ def tile_bytes_per_step(M, S, dim, physics):
     per shot      : u gather, 4c B per mesh slot (tile halos included);
                     accumulator 4c B per slot read (not its first colour) and
                     per slot written (not its last colour); per node 4 B
-                    inverse mass + c (8 B read w, kc + 16 B write u, w, a_,
-                    kc), w = v + dt/2 a_ being the Newmark state the update
-                    reads and kc the packed pair of bf16 Kahan compensations
-                    of u and w
+                    inverse mass + c (8 B read w, kc + 12 B write u, w, kc),
+                    w = v + dt/2 a_ being the Newmark state the update reads
+                    and kc the packed pair of bf16 Kahan compensations of u
+                    and w (a_, read out only, is stored once per graph block)
     """
     c = M.c
     per_tile = 4 * M.tile_slots + 4 * M.ntiles * M.tile_elems * elem_floats(dim, physics)
     per_shot = (4 * c * M.tile_valid_slots + 4 * c * (M.tile_acc_reads + M.tile_acc_writes)
-                + M.G * (4 + 24 * c))
+                + M.G * (4 + 20 * c))
     return per_tile + S * per_shot
 
 

@@ -446,7 +446,7 @@ static int enqueue_receivers(sem_batch *b, int koff) {
   return 0;
 }
 
-int enqueue_fused_step(sem_batch *b, int koff, int ku_only) {
+int enqueue_fused_step(sem_batch *b, int koff, int ku_only, int last) {
   // timing: one event pair around all colour launches of the step (a pair
   // per launch would add its own overhead to short launches)
   const bool timed = b->timing && !ku_only && !b->launches.empty();
@@ -467,6 +467,8 @@ int enqueue_fused_step(sem_batch *b, int koff, int ku_only) {
     a.epb = b->epb;
     a.koff = koff;
     a.ku_only = ku_only;
+    a.ap_koff = last;
+    a.pad = 0;
     auto go = [&]() {
       return L.tile ? launch_tile(L.dim, L.order, L.physics, L.T, L.grid, a, b->stream)
                     : launch_fused(L.dim, L.order, L.physics, L.gmode, L.nitems, b->epb, L.nd, L.c, a, b->stream);
@@ -535,7 +537,7 @@ int capture_graph(sem_batch *b, int steps) {
   def = nullptr;
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
-  for (int k = 0; k < steps && !rc; ++k) rc = enqueue_fused_step(b, k, 0);
+  for (int k = 0; k < steps && !rc; ++k) rc = enqueue_fused_step(b, k, 0, steps - 1);
   if (!rc) rc = launch_advance((int)b->groups.size(), b->d_groups, steps, b->stream);
   hipGraph_t gdef = nullptr;
   hipError_t e = hipStreamEndCapture(b->stream, &gdef);
@@ -564,10 +566,10 @@ static double time_steps(sem_batch *b, int reps) {
   };
   for (auto &e : ev)
     if (hipEventCreate(&e) != hipSuccess) return done(-1);
-  if (enqueue_fused_step(b, 0, 0)) return done(-1);
+  if (enqueue_fused_step(b, 0, 0, 0)) return done(-1);
   (void)hipEventRecord(ev[0], b->stream);
   for (int r = 0; r < reps; ++r) {
-    if (enqueue_fused_step(b, 0, 0)) return done(-1);
+    if (enqueue_fused_step(b, 0, 0, 0)) return done(-1);
     (void)hipEventRecord(ev[r + 1], b->stream);
   }
   if (hipEventSynchronize(ev[reps]) != hipSuccess) return done(-1);
@@ -919,7 +921,7 @@ int sem_batch_run(sem_batch *b, int64_t nsteps) {
         left -= k;
       } else {
         const int k = (int)std::min<int64_t>(left, b->block_steps);
-        for (int j = 0; j < k; ++j) SEM_TRY(enqueue_fused_step(b, j, 0));
+        for (int j = 0; j < k; ++j) SEM_TRY(enqueue_fused_step(b, j, 0, k - 1));
         SEM_TRY(launch_advance((int)b->groups.size(), b->d_groups, k, b->stream));
         left -= k;
       }
@@ -1033,7 +1035,7 @@ int sem_mesh_apply_stiffness(sem_mesh *m, int path, const float *u, float *ku) {
   if (!rc) rc = hip_check(hipMemcpyAsync(b->h_groups[0].u, uin.data(), len * 4, hipMemcpyHostToDevice, b->stream), "upload");
   if (!rc) {
     if (path == SEM_PATH_FUSED) {
-      rc = enqueue_fused_step(b, 0, 1);
+      rc = enqueue_fused_step(b, 0, 1, 0);
     } else {
       GroupDev &h = b->h_groups[0];
       rc = launch_plain_gather(m->E, m->nd, m->c, m->G, m->d_conn, h.u, b->d_uloc, b->stream);

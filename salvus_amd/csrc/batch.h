@@ -144,7 +144,8 @@ void free_group(Group &g);
 // Uploads the group table, receivers and launch plan of the current groups.
 int finalize_groups(sem_batch *b, int epb_hint);
 void drop_graph(sem_batch *b);
-int enqueue_fused_step(sem_batch *b, int koff, int ku_only);
+// One fused step at offset koff of a block whose last offset is `last`.
+int enqueue_fused_step(sem_batch *b, int koff, int ku_only, int last);
 // Default elements per block of the element kernel for nd nodes per element.
 int default_epb(int nd);
 int enqueue_plain_step(sem_batch *b);

@@ -595,7 +595,8 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
             kahan_add(w, kw, G.dt * a);
             kahan_add(u, ku, G.dt * w);
           }
-          gu[idx] = u; gw[idx] = w; gap[idx] = a; gkc[idx] = pack_comp(ku, kw);
+          gu[idx] = u; gw[idx] = w; gkc[idx] = pack_comp(ku, kw);
+          if (A.koff == A.ap_koff || step + 1 >= G.nsteps) gap[idx] = a;  // read out only
         }
       }
       // own slot only: every thread's flux stage has read su (barrier above)

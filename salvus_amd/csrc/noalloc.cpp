@@ -131,6 +131,8 @@ extern "C" int sem_mesh_apply_stiffness_noalloc(sem_mesh *m, const float *u, flo
   a.epb = epb;
   a.koff = 0;
   a.ku_only = 1;
+  a.ap_koff = 0;
+  a.pad = 0;
   for (int c = 0; c < m->ncol; ++c) {
     const int64_t first = m->col_start[c], count = m->col_start[c + 1] - first;
     if (count <= 0) continue;

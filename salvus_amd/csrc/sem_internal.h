@@ -220,6 +220,9 @@ struct FusedArgs {
   int32_t epb;
   int32_t koff;      // step offset within the captured block
   int32_t ku_only;   // 1: accumulate -K u into acc on every colour (no update)
+  int32_t ap_koff;   // a_ (read out only) is stored at this step offset of the
+                     // block -- its last -- and at a shot's last step
+  int32_t pad;
 };
 
 struct PlainArgs {

@@ -337,6 +337,9 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
   const gptr<const float> gmi = glb(G.mi);
   const uint32_t GG = (uint32_t)G.G, Sg = (uint32_t)G.S;
   const bool q3 = (C >= 2) && (G.quirks & SEM_QUIRK_NEWMARK3D);  // {x, y, x}: x twice, z never
+  // a_ is only read out (v = w - dt/2 a_, a_ itself): stored at the block's
+  // last step and at the shot's last step, where a read-out can follow
+  const bool wap = A.koff == A.ap_koff || step + 1 >= G.nsteps;
 
   __syncthreads();  // tile data and the first shot's lattice staged
   const int nts = sS[4 * kTileMaxSrc];
@@ -646,7 +649,8 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
           kahan(w, kw, G.dt * a);
           kahan(u, ku, G.dt * w);
         }
-        gu[idx] = u; gw[idx] = w; gap[idx] = a; gkc[idx] = pack_comp(ku, kw);
+        gu[idx] = u; gw[idx] = w; gkc[idx] = pack_comp(ku, kw);
+        if (wap) gap[idx] = a;
       }
     };
 #pragma unroll

@@ -55,7 +55,7 @@ def test_byte_models_on_a_hand_case():
     # element formulation: 4 nodes x (4 + 12) + 4 x (3 factors + 4 vertex vp) + 4 x (4 + 36)
     assert bench.element_bytes_per_step(M, 1, 2, 0) == 4 * 16 + 4 * 7 + 4 * 40
     # tile algorithm: 4 words + 4 x 7 element floats + (4 u + 4 x (4 mi + 8 read w, kc
-    # + 16 write u, w, a_, kc))
-    assert bench.tile_bytes_per_step(M, 1, 2, 0) == 16 + 28 + 16 + 4 * 28
+    # + 12 write u, w, kc))
+    assert bench.tile_bytes_per_step(M, 1, 2, 0) == 16 + 28 + 16 + 4 * 24
     # S shots scale the per-shot parts only
-    assert bench.tile_bytes_per_step(M, 3, 2, 0) == 16 + 28 + 3 * (16 + 4 * 28)
+    assert bench.tile_bytes_per_step(M, 3, 2, 0) == 16 + 28 + 3 * (16 + 4 * 24)
