@@ -177,10 +177,10 @@ def main():
               "--gpus %d is not supported (use --gpus 1)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    import math
-    # graph blocks that divide both the warm-up and the timed steps, so the
-    # timed region replays graphs captured during the warm-up
-    os.environ["SEM_GRAPH_STEPS"] = str(min(50, math.gcd(args.warmup, args.steps) or 50))
+    # graph blocks of the timed steps (at most 50): the block graph is
+    # captured when the batch is prepared and the warm-up runs its own tail
+    # graph, so the timed region is whole-block replays
+    os.environ["SEM_GRAPH_STEPS"] = str(max(1, min(50, args.steps)))
     from salvus_amd import sem, workloads
 
     timing_steps = 10

@@ -33,6 +33,7 @@ extern "C" {
 typedef struct sem_mesh sem_mesh;
 typedef struct sem_batch sem_batch;
 typedef struct sem_server sem_server;
+typedef struct sem_coupling sem_coupling;
 
 enum { SEM_ACOUSTIC = 0, SEM_ELASTIC = 1 };
 
@@ -216,6 +217,39 @@ int sem_movie_write_mesh_h5(const char *path, int dim, int64_t num_verts, const 
  * appended (else -1 and nothing is written).  Names hold no '/'. */
 int sem_movie_write_h5(const char *path, int append, int nfields, const char *const *names,
                        int64_t n, const double *values, double time);
+/* ------------------------------------------------------------ coupling
+ * 2-D fluid-solid coupling (AcousticToElastic2D.cpp:17-60,
+ * ElasticToAcoustic.cpp:17-41; coupled elements and edge neighbours,
+ * Mesh.cpp:480-521; edge quadrature, TensorQuad.cpp:400-515).  The fluid
+ * (acoustic) and the solid (Elastic2D) elements are two meshes whose cells
+ * come from one quad mesh (shared vertices have identical coordinates).
+ * The coupling holds:
+ *  - the combined inverse mass of nodes on both meshes (the reference
+ *    assembles one mass over all elements, Order2Newmark.cpp:32-66);
+ *  - the fluid load: over every edge of a fluid element with a vertex on a
+ *    solid element that has a neighbouring element (either physics), the
+ *    edge integral of the solid velocity . the element's outward normal
+ *    (ElasticToAcoustic.cpp:27-41; 0 where the edge node is not a solid
+ *    node);
+ *  - the solid load: over every edge of a solid element with a vertex on a
+ *    fluid element whose neighbour is fluid, -rho_0 times the edge integral
+ *    of the fluid velocity, added to both the x and the y row
+ *    (AcousticToElastic2D.cpp:49-60), rho_0 the mean of the neighbour's
+ *    vertex RHO (fluid_vertex_rho: fluid cells x 4).
+ * Velocities are those after the previous step.  The meshes themselves are
+ * not changed. */
+int sem_coupling_create(sem_coupling **coupling, sem_mesh *fluid, sem_mesh *solid,
+                        const double *fluid_vertex_rho);
+int sem_coupling_destroy(sem_coupling *coupling);
+/* info[0..3] = {nodes on both meshes, fluid target nodes, solid target nodes,
+ *               coupling terms} */
+int sem_coupling_info(const sem_coupling *coupling, int64_t *info);
+/* Couples two shots of a batch before sem_batch_prepare: fluid_shot on the
+ * coupling's fluid mesh, solid_shot on its solid mesh, with the same dt and
+ * step count; both advance as one coupled problem (each becomes a group of
+ * its own, a_ is stored every step). */
+int sem_batch_couple(sem_batch *batch, sem_coupling *coupling, int fluid_shot, int solid_shot);
+
 /* Launch configuration chosen: cfg[0..4] = {elems/block, shots/group,
  * graph block steps, groups, groups on the tile kernel (the others take the
  * element kernel)}. */

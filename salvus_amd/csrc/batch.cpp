@@ -31,6 +31,8 @@ void free_group(Group &g) {
   dfree(g.d_load);
   dfree(g.d_step);
   dfree(g.d_seis);
+  dfree(g.ext);
+  dfree(g.mi_own);
   g.recs.clear();
   g.rec_nodes.clear();
   g.rec_w.clear();
@@ -236,6 +238,23 @@ int setup_group(sem_batch *b, int gi) {
   h.dt = (float)g.dt;
   h.acl = (float)((1.0 / 2.0) * g.dt);           // Order2Newmark.cpp:89
   h.quirks = m->quirks;
+  // a coupled shot (a group of its own): external load and combined mass
+  {
+    const Shot &s0 = b->shots[g.shots[0]];
+    if (s0.coupled >= 0) {
+      const std::vector<float> &mc = coupled_mass(b->couples[s0.coupled], m);
+      std::vector<float> mo(m->G);
+      for (int64_t i = 0; i < m->G; ++i) mo[field_pos(g, i)] = mc[i];
+      SEM_TRY(dalloc(&g.mi_own, (size_t)m->G, &g.bytes));
+      SEM_TRY(dalloc(&g.ext, len, &g.bytes));
+      SEM_TRY(launch_fill_f32((int64_t)len, g.ext, 0.f, b->stream));
+      HIPCHK(hipMemcpyAsync(g.mi_own, mo.data(), mo.size() * 4, hipMemcpyHostToDevice, b->stream));
+      HIPCHK(hipStreamSynchronize(b->stream));
+      h.mi = g.mi_own;
+      h.ext = g.ext;
+      h.ap_always = 1;
+    }
+  }
   if ((int)b->h_groups.size() <= gi) b->h_groups.resize(gi + 1);
   b->h_groups[gi] = h;
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -256,6 +275,7 @@ static void release_plan(sem_batch *b) {
 
 static void batch_release(sem_batch *b) {
   drop_graph(b);
+  free_couples(b);
   for (auto &g : b->groups) free_group(g);
   b->groups.clear();
   b->h_groups.clear();
@@ -353,6 +373,7 @@ static int build_plan(sem_batch *b) {
 int finalize_groups(sem_batch *b, int epb_hint) {
   drop_graph(b);
   release_plan(b);
+  SEM_TRY(setup_couples(b));
   // group table
   const size_t ng = b->groups.size();
   if (ng > b->d_groups_cap) {
@@ -447,6 +468,7 @@ static int enqueue_receivers(sem_batch *b, int koff) {
 }
 
 int enqueue_fused_step(sem_batch *b, int koff, int ku_only, int last) {
+  if (!ku_only) SEM_TRY(enqueue_couples(b));  // coupling loads from the velocities after the last step
   // timing: one event pair around all colour launches of the step (a pair
   // per launch would add its own overhead to short launches)
   const bool timed = b->timing && !ku_only && !b->launches.empty();
@@ -481,6 +503,7 @@ int enqueue_fused_step(sem_batch *b, int koff, int ku_only, int last) {
 }
 
 int enqueue_plain_step(sem_batch *b) {
+  SEM_TRY(enqueue_couples(b));
   for (int gi = 0; gi < (int)b->groups.size(); ++gi) {
     Group &g = b->groups[gi];
     GroupDev &h = b->h_groups[gi];
@@ -517,8 +540,8 @@ int enqueue_plain_step(sem_batch *b) {
     for (int c = 0; c < m->ncol; ++c)
       SEM_TRY(launch_plain_scatter(m->col_start[c], m->col_start[c + 1] - m->col_start[c], m->nd, m->c,
                                    m->G, m->d_conn, b->d_aloc, h.acc, b->stream));
-    SEM_TRY(launch_plain_update(m->G, m->c, m->quirks, h.dt, m->d_mi, h.acc, h.u, h.w, h.ap, h.kc,
-                                b->stream));
+    SEM_TRY(launch_plain_update(m->G, m->c, m->quirks, h.dt, h.mi, h.acc, h.ext, h.u, h.w, h.ap,
+                                h.kc, b->stream));
   }
   SEM_TRY(enqueue_receivers(b, 0));
   SEM_TRY(launch_advance((int)b->groups.size(), b->d_groups, 1, b->stream));
@@ -582,11 +605,14 @@ static double time_steps(sem_batch *b, int reps) {
   return done(best);
 }
 
-using Bucket = std::map<std::tuple<sem_mesh *, int64_t, double>, std::vector<int>>;
+// key: mesh, nsteps, dt and (coupled shots, which form groups of their own)
+// the shot index
+using Bucket = std::map<std::tuple<sem_mesh *, int64_t, double, int>, std::vector<int>>;
 static Bucket buckets_of(const sem_batch *b) {
   Bucket bk;
   for (int i = 0; i < (int)b->shots.size(); ++i)
-    bk[std::make_tuple(b->shots[i].mesh, b->shots[i].nsteps, b->shots[i].dt)].push_back(i);
+    bk[std::make_tuple(b->shots[i].mesh, b->shots[i].nsteps, b->shots[i].dt,
+                       b->shots[i].coupled >= 0 ? i : -1)].push_back(i);
   return bk;
 }
 
@@ -892,6 +918,11 @@ int sem_batch_prepare(sem_batch *b, int path, int epb, int spg) {
     }
   }
   b->spg = max_s;
+  // the full-block graph is captured here (capture runs nothing), so the
+  // first run's whole blocks are already replays
+  const char *ng = getenv("SEM_NO_GRAPH");
+  if (path == SEM_PATH_FUSED && !(ng && ng[0] == '1') && (rc = capture_graph(b, b->block_steps)))
+    return fail(rc);
   if ((rc = hip_check(hipStreamSynchronize(b->stream), "sync"))) return fail(rc);
   b->prepared = true;
   return 0;

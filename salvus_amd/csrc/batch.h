@@ -48,6 +48,7 @@ struct Shot {
   std::vector<double> init[3];
   int group = -1, slot = 0;
   int64_t seis_off = 0;  // within the group's seismogram buffer
+  int coupled = -1;      // index of its coupling pair in the batch, or -1
 };
 
 // Shots sharing a mesh, nsteps and dt, advanced in lockstep; fields are
@@ -71,7 +72,36 @@ struct Group {
   std::vector<uint32_t> rec_nodes;
   std::vector<float> rec_w;
   int rec_key = 0;  // components | nodes per element << 8
+  float *ext = nullptr;    // coupled shots: external load added at the first colour
+  float *mi_own = nullptr; // coupled shots: the combined inverse mass (group order)
   size_t bytes = 0;  // device bytes held
+};
+
+// One coupled fluid-solid pair of shots of a batch: the coupling terms in
+// the field numbering of the two groups (each coupled shot is a group of its
+// own), evaluated before every step's colour launches.
+struct CoupleDev {
+  int32_t nf, ns;             // fluid and solid target nodes
+  const int32_t *f_tgt;       // fluid field index of each fluid target
+  const int32_t *f_ptr;       // nf + 1 term offsets
+  const int32_t *f_src;       // solid x field index of each term (y: + G_s)
+  const float *f_wx, *f_wy;   // half length x GLL weight x outward normal
+  const int32_t *s_tgt;       // solid x field index of each solid target (y: + G_s)
+  const int32_t *s_ptr;
+  const int32_t *s_src;       // fluid field index of each term
+  const float *s_coef;        // -rho_0 x half length x GLL weight
+  int64_t Gs;                 // solid nodes (y component offset)
+  const float *wf, *apf;      // fluid state w, a_
+  const float *ws, *aps;      // solid state w, a_ (x then y)
+  float *extf, *exts;         // external loads of the two shots
+  float aclf, acls;           // dt/2 of the two shots
+};
+
+struct CouplePair {
+  sem_coupling *cp = nullptr;
+  int fluid_shot = -1, solid_shot = -1;
+  void *dev = nullptr;        // device arrays of the terms
+  CoupleDev h{};
 };
 
 struct LaunchSet {
@@ -119,6 +149,7 @@ struct sem_batch {
   int64_t kernel_launches = 0;
   std::vector<hipEvent_t> ev;  // timing: event pairs around timed kernels
   int64_t ev_launches = 0;       // kernel launches the event pairs cover
+  std::vector<sem::CouplePair> couples;  // coupled fluid-solid shot pairs
 };
 
 namespace sem {
@@ -152,5 +183,13 @@ int enqueue_plain_step(sem_batch *b);
 // Captures (once) the graph of k steps: k == block_steps, or one cached
 // shorter tail block.
 int capture_graph(sem_batch *b, int k);
+// Coupling pairs (coupling.cpp): device terms of every pair after the groups
+// are set up, the coupling kernel of every pair before a step's launches,
+// and their release.
+int setup_couples(sem_batch *b);
+// The pair's combined inverse mass (DMPlex order) of one of its meshes.
+const std::vector<float> &coupled_mass(const CouplePair &p, const sem_mesh *m);
+int enqueue_couples(sem_batch *b);
+void free_couples(sem_batch *b);
 
 }  // namespace sem

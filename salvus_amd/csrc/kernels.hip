@@ -545,7 +545,7 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
       for (int c = 0; c < C; ++c) {
         const int64_t idx = ((int64_t)s * C + c) * G.G + g;
         if (s == 0) su[c * S::ND + p] = gu[idx];
-        pacc[c] = first ? 0.f : gacc[idx];
+        pacc[c] = first ? (G.ext ? glb(G.ext)[idx] : 0.f) : gacc[idx];
         pw[c] = fin ? gw[idx] : 0.f;
         pkc[c] = fin ? gkc[idx] : 0u;
       }
@@ -575,7 +575,7 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         const int64_t idx = ((int64_t)s * C + c) * G.G + g;
-        const float t = first ? f[c] : pacc[c] + f[c];
+        const float t = (first && !G.ext) ? f[c] : pacc[c] + f[c];
         if (!fin) gacc[idx] = t;
         else f[c] = t;
       }
@@ -596,7 +596,7 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
             kahan_add(u, ku, G.dt * w);
           }
           gu[idx] = u; gw[idx] = w; gkc[idx] = pack_comp(ku, kw);
-          if (A.koff == A.ap_koff || step + 1 >= G.nsteps) gap[idx] = a;  // read out only
+          if (A.koff == A.ap_koff || step + 1 >= G.nsteps || G.ap_always) gap[idx] = a;  // read out only
         }
       }
       // own slot only: every thread's flux stage has read su (barrier above)
@@ -690,7 +690,8 @@ __global__ void k_plain_scatter(int64_t first, int64_t count, int nd, int C, int
 
 __global__ void k_plain_update(int64_t G, int C, int quirks, float dt,
                                const float *__restrict__ mi, const float *__restrict__ acc,
-                               float *__restrict__ u, float *__restrict__ w, float *__restrict__ ap,
+                               const float *__restrict__ ext, float *__restrict__ u,
+                               float *__restrict__ w, float *__restrict__ ap,
                                uint32_t *__restrict__ kc) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= G) return;
@@ -699,7 +700,7 @@ __global__ void k_plain_update(int64_t G, int C, int quirks, float dt,
   for (int c = 0; c < C; ++c) {
     if (q3 && c == 2) continue;
     int64_t i = (int64_t)c * G + g;
-    float a = m * acc[i];
+    float a = m * (ext ? acc[i] + ext[i] : acc[i]);
     const uint32_t k2 = kc[i];
     float ww = w[i], kww = comp_v(k2), uu = u[i], kuu = comp_u(k2);
     int reps = (q3 && c == 0) ? 2 : 1;
@@ -916,8 +917,8 @@ int launch_plain_scatter(int64_t first, int64_t count, int nd, int C, int64_t G,
 }
 
 int launch_plain_update(int64_t G, int C, int quirks, float dt, const float *mi, const float *acc,
-                        float *u, float *w, float *ap, uint32_t *kc, hipStream_t st) {
-  k_plain_update<<<nblocks(G, 256), 256, 0, st>>>(G, C, quirks, dt, mi, acc, u, w, ap, kc);
+                        const float *ext, float *u, float *w, float *ap, uint32_t *kc, hipStream_t st) {
+  k_plain_update<<<nblocks(G, 256), 256, 0, st>>>(G, C, quirks, dt, mi, acc, ext, u, w, ap, kc);
   return last_launch("plain update");
 }
 

@@ -181,6 +181,12 @@ struct GroupDev {
   const uint16_t *tile_slot;    // lattice slot of each tile word
   const uint32_t *tile_words_lat;  // per tile, in lattice-slot order
   const float *telem;     // per element in tile order: affine factors, then vertex material
+  // coupled shots (fluid-solid pairs): external load added to a node's sum
+  // at its first colour, null otherwise; a_ stored every step (the coupling
+  // reads the velocities v = w - dt/2 a_ after each step)
+  const float *ext;
+  int32_t ap_always;
+  int32_t pad2;
 };
 
 // Work item of a colour launch: EPB elements of one group (element kernel),
@@ -284,7 +290,7 @@ int launch_plain_scatter(int64_t first, int64_t count, int nd, int C, int64_t G,
                          const uint32_t *conn, const float *aloc, float *acc, hipStream_t st);
 // Newmark update in the (u, w = v + dt/2 a_) form: w += dt a, u += dt w.
 int launch_plain_update(int64_t G, int C, int quirks, float dt, const float *mi, const float *acc,
-                        float *u, float *w, float *ap, uint32_t *kc, hipStream_t st);
+                        const float *ext, float *u, float *w, float *ap, uint32_t *kc, hipStream_t st);
 int launch_receivers(const RecArgs &a, int C, hipStream_t st);
 // Advances every group's step counter by k (clamped at its nsteps).
 int launch_advance(int ngroups, const GroupDev *groups, int64_t k, hipStream_t st);

@@ -339,7 +339,9 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
   const bool q3 = (C >= 2) && (G.quirks & SEM_QUIRK_NEWMARK3D);  // {x, y, x}: x twice, z never
   // a_ is only read out (v = w - dt/2 a_, a_ itself): stored at the block's
   // last step and at the shot's last step, where a read-out can follow
-  const bool wap = A.koff == A.ap_koff || step + 1 >= G.nsteps;
+  const bool wap = A.koff == A.ap_koff || step + 1 >= G.nsteps || G.ap_always;
+  const gptr<const float> gext = glb(G.ext);  // coupled shots: external load
+  const bool has_ext = G.ext != nullptr;
 
   __syncthreads();  // tile data and the first shot's lattice staged
   const int nts = sS[4 * kTileMaxSrc];
@@ -566,7 +568,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         const uint32_t idx = (s * C + c) * GG + g;
-        in.prev[c] = (valid && !first) ? gacc[idx] : 0.f;
+        in.prev[c] = !valid ? 0.f : !first ? gacc[idx] : has_ext ? gext[idx] : 0.f;
         in.vw[c] = fin ? gw[idx] : 0.f;
         in.kc[c] = fin ? gkc[idx] : 0u;
       }
@@ -635,7 +637,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
       for (int c = 0; c < C; ++c) {
         const uint32_t idx = (s * C + c) * GG + g;
         const float tot = sK[(e * C + c) * ND + p];
-        const float t = first ? tot : in.prev[c] + tot;
+        const float t = (first && !has_ext) ? tot : in.prev[c] + tot;
         if (!fin) {
           gacc[idx] = t;
           continue;

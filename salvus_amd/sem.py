@@ -40,7 +40,8 @@ EXPORTS = (
     "sem_server_submit", "sem_server_pump", "sem_server_request", "sem_server_result",
     "sem_mesh_autotune", "sem_shot_add_file_source", "sem_receivers_write_h5",
     "sem_source_read_h5", "sem_movie_write_h5", "sem_mesh_apply_stiffness_noalloc",
-    "sem_server_cancel", "sem_movie_write_mesh_h5",
+    "sem_server_cancel", "sem_movie_write_mesh_h5", "sem_coupling_create",
+    "sem_coupling_destroy", "sem_coupling_info", "sem_batch_couple",
 )
 SOURCE_DOUBLES = 19
 
@@ -108,6 +109,10 @@ def lib():
         L.sem_server_cancel.argtypes = [_VP, C.c_int]
         L.sem_movie_write_mesh_h5.argtypes = [C.c_char_p, C.c_int, C.c_int64, _D, C.c_int64, _I32,
                                               C.c_int64, _D]
+        L.sem_coupling_create.argtypes = [C.POINTER(_VP), _VP, _VP, _D]
+        L.sem_coupling_destroy.argtypes = [_VP]
+        L.sem_coupling_info.argtypes = [_VP, _I64]
+        L.sem_batch_couple.argtypes = [_VP, _VP, C.c_int, C.c_int]
         L.sem_mesh_autotune.argtypes = [_VP, C.c_int, C.c_int, _I64]
         L.sem_shot_add_file_source.argtypes = [_VP, C.c_int, _D, C.c_int, C.c_int64, _D]
         L.sem_receivers_write_h5.argtypes = [C.c_char_p, C.c_int, C.c_int,
@@ -320,6 +325,39 @@ class Mesh:
         return int(v[0])
 
 
+class Coupling:
+    """2-D fluid-solid coupling of an acoustic and an elastic mesh cut from
+    one quad mesh (the coupled elements of Mesh.cpp:480-521 with the edge
+    integrals of ElasticToAcoustic.cpp:17-41 and AcousticToElastic2D.cpp:17-60)
+    and the combined inverse mass at their shared nodes
+    (Order2Newmark.cpp:32-66).  `fluid_vertex_rho` holds RHO at the four
+    vertices of every fluid cell (the density rho_0 of AcousticToElastic2D is
+    their mean).  Pair a fluid and a solid shot with Batch.couple."""
+
+    def __init__(self, fluid, solid, fluid_vertex_rho):
+        rho = np.ascontiguousarray(fluid_vertex_rho, dtype=np.float64).reshape(-1)
+        if rho.size != 4 * fluid.E:
+            raise SemError("fluid_vertex_rho needs 4 values per fluid cell")
+        self.fluid, self.solid = fluid, solid
+        self._h = _VP()
+        _check(lib().sem_coupling_create(C.byref(self._h), fluid.handle, solid.handle, _d(rho)))
+        info = np.zeros(4, dtype=np.int64)
+        _check(lib().sem_coupling_info(self._h, _l(info)))
+        self.shared_nodes, self.fluid_targets, self.solid_targets, self.terms = (int(x) for x in info)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sem_coupling_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+
 class Batch:
     """Shots advanced together (Problem::initializeElements source/receiver
     attach, Problem.cpp:34-142; the step loop of Main.cpp:45-66)."""
@@ -329,6 +367,7 @@ class Batch:
         _check(lib().sem_batch_create(C.byref(self._h), device))
         self.meshes = []
         self.shots = []  # (mesh, nsteps, nrec)
+        self.couplings = []  # kept alive while the batch refers to them
 
     def close(self):
         if getattr(self, "_h", None):
@@ -344,6 +383,12 @@ class Batch:
         self.meshes.append(mesh)
         self.shots.append([mesh, nsteps, 0])
         return k.value
+
+    def couple(self, coupling, fluid_shot, solid_shot):
+        """Couples a fluid and a solid shot of this batch (same dt and step
+        count) through `coupling`; before prepare."""
+        _check(lib().sem_batch_couple(self._h, coupling.handle, fluid_shot, solid_shot))
+        self.couplings.append(coupling)
 
     def add_source(self, shot, loc, amp, f0, t0, direction=None):
         m = self.shots[shot][0]
