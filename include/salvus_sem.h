@@ -90,7 +90,9 @@ int sem_mesh_destroy(sem_mesh *mesh);
  *                affine (1/0), device bytes held, dim, tiles (0 = element
  *                kernel), tile shape Tx*10000 + Ty*100 + Tz, tile lattice
  *                slots, slots on the mesh, accumulator reads and writes per
- *                shot-step, elements per tile (padded), 0} */
+ *                shot-step, elements per tile (padded), diagonal
+ *                geometry (1: every element's J^-1 is diagonal and the tile
+ *                kernel skips the off-diagonal terms)} */
 int sem_mesh_info(const sem_mesh *mesh, int64_t *info);
 /* Tile-shape autotuner (elements per workgroup of the tile kernel): for
  * every compiled tile shape of the mesh's order and physics, times 2 x `reps`

@@ -235,6 +235,14 @@ int sem::upload_tiles(sem_mesh *m) {
   if (!rc) rc = hip_check(hipMemcpyAsync(tg.data(), d_tg, tg.size() * 4, hipMemcpyDeviceToHost, m->stream), "download");
   if (!rc) rc = hip_check(hipStreamSynchronize(m->stream), "sync");
   if (!rc) {
+    // off-diagonal entries: J^-1 row-major (elastic), packed symmetric G (acoustic)
+    static const int off_e2[] = {1, 2}, off_e3[] = {1, 2, 3, 5, 6, 7}, off_a2[] = {1}, off_a3[] = {1, 2, 4};
+    const int *off = m->physics == SEM_ACOUSTIC ? (dim == 2 ? off_a2 : off_a3) : (dim == 2 ? off_e2 : off_e3);
+    const int noff = m->physics == SEM_ACOUSTIC ? (dim == 2 ? 1 : 3) : (dim == 2 ? 2 : 6);
+    bool diag = true;
+    for (int64_t i = 0; i < E && diag; ++i)
+      for (int k = 0; k < noff; ++k) diag = diag && tg[i * ng + off[k]] == 0.f;
+    m->tile_diag = diag;
     for (int64_t i = 0; i < E; ++i) {
       std::copy_n(&tg[i * ng], ng, &te[i * ned]);
       std::copy_n(&tp[i * nv * kp], nv * kp, &te[i * ned + ng]);
@@ -476,7 +484,7 @@ int sem_mesh_info(const sem_mesh *m, int64_t *info) {
   info[12] = accr;
   info[13] = accw;
   info[14] = m->tiles.valid ? m->tiles.elems_per_tile : 0;
-  info[15] = 0;
+  info[15] = m->tiles.valid && m->tile_diag;
   return 0;
 }
 

@@ -294,13 +294,17 @@ static void batch_release(sem_batch *b) {
 // Work items per (kernel key, colour): tiles of tiled groups, EPB-element
 // slices of the element kernel's colour ranges otherwise.
 static int build_plan(sem_batch *b) {
-  // key = (tile?, dim, order, physics, geometry mode, tile shape)
+  // key = (tile? | diagonal geometry << 1, dim, order, physics, geometry
+  // mode, tile shape); SEM_TILE_GENERAL=1 keeps the general tile kernel
+  const char *gen = getenv("SEM_TILE_GENERAL");
+  const bool general = gen && gen[0] == '1';
   std::map<std::tuple<int, int, int, int, int, int>, std::vector<int>> keys;
   for (int gi = 0; gi < (int)b->groups.size(); ++gi) {
     sem_mesh *m = b->groups[gi].mesh;
     const bool t = b->groups[gi].tiled;
     const int shape = t ? m->tiles.T[0] * 10000 + m->tiles.T[1] * 100 + m->tiles.T[2] : 0;
-    keys[std::make_tuple((int)t, m->dim, m->order, m->physics, geo_mode(m), shape)].push_back(gi);
+    const int tk = t ? 1 + 2 * (int)(m->tile_diag && !general) : 0;
+    keys[std::make_tuple(tk, m->dim, m->order, m->physics, geo_mode(m), shape)].push_back(gi);
   }
   for (auto &l : b->launches) dfree(l.d_items);
   b->launches.clear();
@@ -343,6 +347,7 @@ static int build_plan(sem_batch *b) {
       if (items.empty()) continue;
       LaunchSet L;
       L.tile = tile;
+      L.diag = (std::get<0>(kv.first) & 2) != 0;
       L.dim = std::get<1>(kv.first);
       L.order = std::get<2>(kv.first);
       L.physics = std::get<3>(kv.first);
@@ -492,7 +497,7 @@ int enqueue_fused_step(sem_batch *b, int koff, int ku_only, int last) {
     a.ap_koff = last;
     a.pad = 0;
     auto go = [&]() {
-      return L.tile ? launch_tile(L.dim, L.order, L.physics, L.T, L.grid, a, b->stream)
+      return L.tile ? launch_tile(L.dim, L.order, L.physics, L.T, L.diag, L.grid, a, b->stream)
                     : launch_fused(L.dim, L.order, L.physics, L.gmode, L.nitems, b->epb, L.nd, L.c, a, b->stream);
     };
     SEM_TRY(go());

@@ -108,6 +108,7 @@ struct LaunchSet {
   int dim, order, physics, nd, c, colour;
   int gmode;  // geometry mode of the element kernel (kGeo*)
   bool tile;
+  bool diag;  // tile kernel: diagonal-geometry variant
   int nitems;
   int T[3];  // tile shape (tile kernel)
   int grid;  // tile kernel: workgroups (one per tile)

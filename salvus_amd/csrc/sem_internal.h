@@ -307,7 +307,7 @@ int tile_kernel_shapes(int dim, int order, int physics, int *shapes, int max);
 // Readies the kernel of this shape (LDS limit) and returns its workgroups per CU.
 int tile_kernel_prepare(int dim, int order, int physics, const int *T, int *per_cu);
 // One workgroup per tile: grid = a.nitems.
-int launch_tile(int dim, int order, int physics, const int *T, int grid, const FusedArgs &a,
+int launch_tile(int dim, int order, int physics, const int *T, bool diag, int grid, const FusedArgs &a,
                 hipStream_t st);
 
 }  // namespace sem
@@ -343,6 +343,7 @@ struct sem_mesh {
   uint16_t *d_tile_slot = nullptr;
   uint32_t *d_tile_words_lat = nullptr;
   float *d_telem = nullptr, *d_mi_int = nullptr;
+  bool tile_diag = false;  // every element's affine factors diagonal (axis-aligned lattice)
   size_t bytes = 0;       // device bytes held (tile path included)
   size_t tile_bytes = 0;  // of which the tile path
 };

@@ -173,6 +173,12 @@ struct TS {
   static constexpr int LDS = O_SL + (SLOT_LDS ? (NTL + 1) / 2 : 0);
 };
 
+// a + b never fused with a product feeding it (per-operation contract flag)
+__device__ __forceinline__ float add_unfused(float a, float b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+
 __device__ __forceinline__ void kahan(float &s, float &c, float x) {
   const float y = x - c;
   const float t = s + y;
@@ -201,7 +207,11 @@ __device__ __forceinline__ float p1_face(int v, float a, float b) {
   return 0.25f * (1.f + hs[v][0] * a) * (1.f + hs[v][1] * b);
 }
 
-template <int DIM, int N, int PHYS, int TX, int TY, int TZ, bool HO>
+// DG: every element's J^-1 (and, acoustic, its G = det J J^-1 J^-T) is
+// diagonal (axis-aligned lattices), so the contractions keep their diagonal
+// terms only; the terms dropped are exact zeros, so results are those of the
+// general kernel.
+template <int DIM, int N, int PHYS, int TX, int TY, int TZ, bool HO, bool DG>
 __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
     __attribute__((amdgpu_waves_per_eu(TS<DIM, N, PHYS, TX, TY, TZ, HO>::MINW))) k_tile(FusedArgs A) {
   using T = TS<DIM, N, PHYS, TX, TY, TZ, HO>;
@@ -430,7 +440,14 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
             const float vp = fmaf(mB[0], tk, mA[0]);
             const float sc = wl * wk * vp * vp;  // Scalar.cpp:45
             const float *Gs = ed;
-            if constexpr (DIM == 2) {
+            if constexpr (DG && DIM == 2) {
+              F[0][0] = sc * (Gs[0] * g[0][0]);
+              F[0][1] = sc * (Gs[2] * g[0][1]);
+            } else if constexpr (DG) {
+              F[0][0] = sc * (Gs[0] * g[0][0]);
+              F[0][NL > 1 ? 1 : 0] = sc * (Gs[3] * g[0][1]);
+              F[0][DIM - 1] = sc * (Gs[5] * g[0][DIM - 1]);
+            } else if constexpr (DIM == 2) {
               F[0][0] = sc * (Gs[0] * g[0][0] + Gs[1] * g[0][1]);
               F[0][1] = sc * (Gs[1] * g[0][0] + Gs[2] * g[0][1]);
             } else {
@@ -462,23 +479,32 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
 #pragma unroll
               for (int a = 0; a < DIM; ++a) {
                 float v = 0.f;
+                if constexpr (DG) {
+                  v = Ji[a * DIM + a] * g[c < C ? c : 0][a];
+                } else {
 #pragma unroll
-                for (int b = 0; b < DIM; ++b) v = fmaf(Ji[a * DIM + b], g[c < C ? c : 0][b], v);
+                  for (int b = 0; b < DIM; ++b) v = fmaf(Ji[a * DIM + b], g[c < C ? c : 0][b], v);
+                }
                 gx[c][a] = v;
               }
             float sig[DIM][DIM];
             if constexpr (DIM == 2) {  // Elastic2D.cpp:53-82
-              const float exx = gx[0][0], eyy = gx[1][1], sh = gx[0][1] + gx[1][0];
-              sig[0][0] = cm[0] * exx + cm[1] * eyy;
-              sig[1][1] = cm[1] * exx + cm[2] * eyy;
+              // shear sums unfused: with diagonal J^-1 the terms are plain
+              // products the compiler could otherwise fuse into the sum, and
+              // the variants would round differently
+              const float exx = gx[0][0], eyy = gx[1][1], sh = add_unfused(gx[0][1], gx[1][0]);
+              // explicit fmaf: the same fused products in every variant
+              sig[0][0] = fmaf(cm[1], eyy, cm[0] * exx);
+              sig[1][1] = fmaf(cm[2], eyy, cm[1] * exx);
               sig[0][1] = sig[1][0] = cm[3] * sh;
             } else {  // Elastic3D.cpp:76-83, :106-118
               constexpr int Y = DIM > 2 ? 2 : 1;
               const float e0 = gx[0][0], e1 = gx[1][1], e2 = gx[Y][Y];
-              const float e3 = gx[1][Y] + gx[Y][1], e4 = gx[0][Y] + gx[Y][0], e5 = gx[0][1] + gx[1][0];
-              sig[0][0] = cm[0] * e0 + cm[1] * e1 + cm[2] * e2;
-              sig[1][1] = cm[1] * e0 + cm[3] * e1 + cm[4] * e2;
-              sig[Y][Y] = cm[2] * e0 + cm[4] * e1 + cm[5] * e2;
+              const float e3 = add_unfused(gx[1][Y], gx[Y][1]), e4 = add_unfused(gx[0][Y], gx[Y][0]);
+              const float e5 = add_unfused(gx[0][1], gx[1][0]);  // unfused, as in 2-D
+              sig[0][0] = fmaf(cm[2], e2, fmaf(cm[1], e1, cm[0] * e0));  // fixed fusion, as in 2-D
+              sig[1][1] = fmaf(cm[4], e2, fmaf(cm[3], e1, cm[1] * e0));
+              sig[Y][Y] = fmaf(cm[5], e2, fmaf(cm[4], e1, cm[2] * e0));
               sig[1][Y] = sig[Y][1] = cm[6] * e3;
               sig[0][Y] = sig[Y][0] = cm[7] * e4;
               sig[0][1] = sig[1][0] = cm[8] * e5;
@@ -488,8 +514,12 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
 #pragma unroll
               for (int a = 0; a < DIM; ++a) {
                 float v = 0.f;
+                if constexpr (DG) {
+                  v = Ji[a * DIM + a] * sig[c][a];
+                } else {
 #pragma unroll
-                for (int b = 0; b < DIM; ++b) v = fmaf(Ji[b * DIM + a], sig[c][b], v);
+                  for (int b = 0; b < DIM; ++b) v = fmaf(Ji[b * DIM + a], sig[c][b], v);
+                }
                 F[c < C ? c : 0][a] = DIM == 3 ? v : wdet * v;
               }
           }
@@ -775,10 +805,13 @@ static int prep_one(int *per_cu) {
   using S = TS<D, N, P, X, Y, Z, HO>;
   const size_t shm = sizeof(float) * (size_t)S::LDS;
   if (shm > 160 * 1024) { set_error("tile does not fit in LDS"); return -1; }
-  const void *fn = (const void *)k_tile<D, N, P, X, Y, Z, HO>;
+  const void *fn = (const void *)k_tile<D, N, P, X, Y, Z, HO, false>;
   if (shm > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-    if (e != hipSuccess) { set_error(std::string("tile LDS attribute: ") + hipGetErrorString(e)); return -2; }
+    const void *fns[2] = {fn, (const void *)k_tile<D, N, P, X, Y, Z, HO, true>};
+    for (const void *f : fns) {
+      hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+      if (e != hipSuccess) { set_error(std::string("tile LDS attribute: ") + hipGetErrorString(e)); return -2; }
+    }
   }
   int nb = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, S::BT, shm);
@@ -834,7 +867,7 @@ int tile_kernel_prepare(int dim, int order, int physics, const int *T, int *per_
 // at 703 tiles per colour on 256 CUs, 4 % slower at 312).  Both variants
 // do the same arithmetic in the same order: results are identical.
 // SEM_TILE_OCCUPANCY=normal|high forces one variant (tests compare them).
-template <int D, int N, int P, int X, int Y, int Z>
+template <int D, int N, int P, int X, int Y, int Z, bool DG>
 static int go_tile(int grid, const FusedArgs &a, hipStream_t st) {
   using S = TS<D, N, P, X, Y, Z, false>;
   const size_t shm = sizeof(float) * (size_t)S::LDS;
@@ -849,23 +882,26 @@ static int go_tile(int grid, const FusedArgs &a, hipStream_t st) {
                                                           : 6 * hi < 5 * lo;  // 1.2 hi < lo
     if (high) {
       using SH = TS<D, N, P, X, Y, Z, true>;
-      hipLaunchKernelGGL((k_tile<D, N, P, X, Y, Z, true>), dim3(grid), dim3(SH::BT),
+      hipLaunchKernelGGL((k_tile<D, N, P, X, Y, Z, true, DG>), dim3(grid), dim3(SH::BT),
                          sizeof(float) * (size_t)SH::LDS, st, a);
       return 0;
     }
   }
-  hipLaunchKernelGGL((k_tile<D, N, P, X, Y, Z, false>), dim3(grid), dim3(S::BT), shm, st, a);
+  hipLaunchKernelGGL((k_tile<D, N, P, X, Y, Z, false, DG>), dim3(grid), dim3(S::BT), shm, st, a);
   return 0;
 }
 
-int launch_tile(int dim, int order, int physics, const int *T, int grid, const FusedArgs &a,
+int launch_tile(int dim, int order, int physics, const int *T, bool diag, int grid, const FusedArgs &a,
                 hipStream_t st) {
   if (a.nitems <= 0 || grid <= 0) return 0;
   int rc = 1;
 #define TLN(D, N, X, Y, Z)                                                                  \
   if (rc == 1 && dim == D && order == N && same_shape(T, X, Y, D == 3 ? Z : 1))             \
-    rc = physics == SEM_ACOUSTIC ? go_tile<D, N, 0, X, Y, Z>(grid, a, st)                   \
-                                 : go_tile<D, N, 1, X, Y, Z>(grid, a, st);
+    rc = physics == SEM_ACOUSTIC                                                            \
+             ? (diag ? go_tile<D, N, 0, X, Y, Z, true>(grid, a, st)                         \
+                     : go_tile<D, N, 0, X, Y, Z, false>(grid, a, st))                       \
+             : (diag ? go_tile<D, N, 1, X, Y, Z, true>(grid, a, st)                         \
+                     : go_tile<D, N, 1, X, Y, Z, false>(grid, a, st));
   SEM_TILE_KERNELS(TLN)
 #undef TLN
   if (rc == 1) { set_error("tile kernel not compiled for this order/shape"); return -1; }

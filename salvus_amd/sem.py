@@ -267,6 +267,7 @@ class Mesh:
         self.tile_slots, self.tile_valid_slots = int(info[10]), int(info[11])
         self.tile_acc_reads, self.tile_acc_writes = int(info[12]), int(info[13])
         self.tile_elems = int(info[14])
+        self.tile_diagonal = bool(info[15])
 
     def autotune(self, reps=5, shots=1):
         """Times every compiled tile shape over `reps` steps of one group of

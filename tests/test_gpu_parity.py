@@ -298,3 +298,21 @@ def test_acoustic_occupancy_variants_identical(dim, tile, monkeypatch):
         out[mode] = (seis, u)
     np.testing.assert_array_equal(out["high"][0], out["normal"][0])
     np.testing.assert_array_equal(out["high"][1], out["normal"][1])
+
+
+@pytest.mark.parametrize("dim,physics", CASES)
+def test_diagonal_geometry_variant_identical(dim, physics, monkeypatch):
+    """Axis-aligned lattices take the tile kernel that keeps only the
+    diagonal of J^-1 (or of G); the terms it drops are exact zeros, so its
+    results are those of the general tile kernel, and both match the
+    oracle."""
+    P = pb.make(dim, physics, 5 if dim == 3 else 10, order=4, seed=41 + dim, nsteps=150)
+    out = {}
+    for general in ("0", "1"):
+        monkeypatch.setenv("SEM_TILE_GENERAL", general)
+        seis, ref, u, u_ref, prob, _ = _run_both(P, sem.PATH_FUSED)
+        assert prob.mesh.ntiles > 0 and prob.mesh.tile_diagonal
+        assert pb.rel_l2(seis, ref) < TOL_RUN and pb.rel_l2(u, u_ref) < TOL_RUN
+        out[general] = (seis, u)
+    np.testing.assert_array_equal(out["0"][0], out["1"][0])
+    np.testing.assert_array_equal(out["0"][1], out["1"][1])
