@@ -275,9 +275,15 @@ int64_t sem_reference_step_count(double duration, double dt, double *dt_out);
  * completion; all active requests advance together, K steps per pump, in the
  * same colour launches; between blocks finished requests are retired and
  * arrived requests admitted in arrival order while the HBM budget (bytes) and
- * the active cap (0 = none) allow.  Times are seconds on the server's clock
- * (sem_server_clock; 0 at creation).  path SEM_PATH_PLAIN with max_active 1
- * is the one-shot-per-launch baseline. */
+ * the active cap (0 = none, counted in requests) allow.  On the fused path
+ * the request at the head of the queue is admitted with later arrived
+ * requests on the same mesh and dt whose step counts lie within a spread of
+ * the largest (sem_server_set_grouping; default up to 4 shots, 10 %): they
+ * form one multi-shot group that steps to its largest count, and each
+ * request's seismograms are copied out at its own count.  Times are seconds
+ * on the server's clock (sem_server_clock; 0 at creation).  path
+ * SEM_PATH_PLAIN with max_active 1 is the one-shot-per-launch baseline (one
+ * shot per group). */
 #define SEM_SOURCE_DOUBLES 19 /* kind (0 force, 1 moment), x, y, z, amp, f0, t0,
                                  dir[3], M[3][3] row-major */
 int sem_server_create(sem_server **server, int device, int path, int64_t hbm_budget,
@@ -292,11 +298,18 @@ int sem_server_submit(sem_server *server, sem_mesh *mesh, int64_t nsteps, double
  *                 (-1: none), device bytes held by active requests} */
 int sem_server_pump(sem_server *server, double *status);
 /* Cancels a request (no reference counterpart): a pending request leaves
- * the queue; an active one
- * is retired at once, its device memory released and its partial results
- * dropped.  Finished or cancelled requests are an error.  The other requests'
- * results do not change (each request's arithmetic is its own). */
+ * the queue; an active one is retired at once and its partial results
+ * dropped, its group's device memory released unless other active requests
+ * share the group (then the shot steps on with it, unobserved, until the
+ * group ends).  Finished or cancelled requests are an error.  The other
+ * requests' results do not change (each request's arithmetic is its own). */
 int sem_server_cancel(sem_server *server, int request);
+/* Multi-shot groups of the fused path: at most shots_per_group requests per
+ * group (1: every request alone), step counts within step_spread (0 <= s <
+ * 1) of the group's largest.  Applies to later admissions. */
+int sem_server_set_grouping(sem_server *server, int shots_per_group, double step_spread);
+/* *shots = shots of the group the request was admitted in (0: not admitted) */
+int sem_server_request_group(const sem_server *server, int request, int *shots);
 /* info[0..5] = {state (0 pending, 1 active, 2 done, 3 failed, 4 cancelled),
  *               arrival, admission, completion, steps done, device bytes} */
 int sem_server_request(const sem_server *server, int request, double *info);

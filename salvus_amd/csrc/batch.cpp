@@ -154,7 +154,9 @@ int setup_group(sem_batch *b, int gi) {
         d.load_off = (int32_t)load.size();
         d.shot = sl;
         if (sp0.kind == 2) {
-          for (int64_t n = 0; n < g.nsteps; ++n) stf.push_back((float)sp0.series[k * sp0.nser + n]);
+          // (a shot shorter than its group's step count: zeros past its end)
+          for (int64_t n = 0; n < g.nsteps; ++n)
+            stf.push_back(n < sp0.nser ? (float)sp0.series[k * sp0.nser + n] : 0.f);
         } else {
           double t = 0.0;
           for (int64_t n = 0; n < g.nsteps; ++n) {
@@ -193,7 +195,9 @@ int setup_group(sem_batch *b, int gi) {
       }
       g.recs.push_back(d);
     }
-    seis += (int64_t)s.recs.size() * m->c * s.nsteps;
+    // rows for every step of the group (a server groups shots whose step
+    // counts differ a little; a shot's own rows are the first s.nsteps)
+    seis += (int64_t)s.recs.size() * m->c * g.nsteps;
   }
   g.seis_len = seis;
   SEM_TRY(dalloc(&g.d_seis, (size_t)std::max<int64_t>(seis, 1), &g.bytes));

@@ -41,7 +41,8 @@ EXPORTS = (
     "sem_mesh_autotune", "sem_shot_add_file_source", "sem_receivers_write_h5",
     "sem_source_read_h5", "sem_movie_write_h5", "sem_mesh_apply_stiffness_noalloc",
     "sem_server_cancel", "sem_movie_write_mesh_h5", "sem_coupling_create",
-    "sem_coupling_destroy", "sem_coupling_info", "sem_batch_couple",
+    "sem_coupling_destroy", "sem_coupling_info", "sem_batch_couple", "sem_server_set_grouping",
+    "sem_server_request_group",
 )
 SOURCE_DOUBLES = 19
 
@@ -107,6 +108,8 @@ def lib():
         L.sem_server_request.argtypes = [_VP, C.c_int, _D]
         L.sem_server_result.argtypes = [_VP, C.c_int, _F]
         L.sem_server_cancel.argtypes = [_VP, C.c_int]
+        L.sem_server_set_grouping.argtypes = [_VP, C.c_int, C.c_double]
+        L.sem_server_request_group.argtypes = [_VP, C.c_int, C.POINTER(C.c_int)]
         L.sem_movie_write_mesh_h5.argtypes = [C.c_char_p, C.c_int, C.c_int64, _D, C.c_int64, _I32,
                                               C.c_int64, _D]
         L.sem_coupling_create.argtypes = [C.POINTER(_VP), _VP, _VP, _D]
@@ -562,14 +565,19 @@ class Server:
     request is one shot; active shots advance together block_steps steps per
     pump(); finished ones are retired and arrived ones admitted in arrival
     order while the HBM budget (bytes, 0 = free memory less a reserve) and
-    max_active (0 = no cap) allow."""
+    max_active (0 = no cap) allow.  On the fused path the head of the queue
+    is admitted with up to shots_per_group - 1 later arrived requests on its
+    mesh and dt whose step counts lie within step_spread of the largest (one
+    multi-shot group)."""
 
     STATES = {0: "pending", 1: "active", 2: "done", 3: "failed", 4: "cancelled"}
 
-    def __init__(self, path=PATH_FUSED, hbm_budget=0, max_active=0, block_steps=32, device=0):
+    def __init__(self, path=PATH_FUSED, hbm_budget=0, max_active=0, block_steps=32, device=0,
+                 shots_per_group=4, step_spread=0.1):
         self._h = _VP()
         _check(lib().sem_server_create(C.byref(self._h), device, path, int(hbm_budget),
                                        max_active, block_steps))
+        _check(lib().sem_server_set_grouping(self._h, int(shots_per_group), float(step_spread)))
         self.requests = []  # (mesh, nsteps, nrec)
 
     def close(self):
@@ -609,12 +617,15 @@ class Server:
     def request(self, rid):
         info = np.zeros(6)
         _check(lib().sem_server_request(self._h, rid, _d(info)))
+        shots = C.c_int(0)
+        _check(lib().sem_server_request_group(self._h, rid, C.byref(shots)))
         return dict(state=self.STATES[int(info[0])], arrival=info[1], admit=info[2], done=info[3],
-                    steps=int(info[4]), bytes=int(info[5]))
+                    steps=int(info[4]), bytes=int(info[5]), group_shots=shots.value)
 
     def cancel(self, rid):
         """A pending request leaves the queue; an active one is retired at
-        once (device memory released, partial results dropped)."""
+        once (partial results dropped; device memory released with its
+        group)."""
         _check(lib().sem_server_cancel(self._h, rid))
 
     def result(self, rid):

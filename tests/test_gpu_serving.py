@@ -157,6 +157,62 @@ def test_cancel_pending_and_active():
         np.testing.assert_array_equal(S.result(ids[k]), alone[k])
 
 
+def test_multi_shot_groups():
+    """Arrived requests on one mesh with step counts within the spread are
+    admitted as one group and share its launches; a shorter one is done at
+    its own count; every result equals the request served alone, and equals
+    a server that keeps every request alone."""
+    Ps = _problems()
+    P, Q = Ps[1], Ps[3]
+    M, MQ = _mesh(P), _mesh(Q)
+    # (mesh, problem, steps, source scale): the first three group (70, 66, 72:
+    # within 10 % of 72), 40 steps is outside the spread, the hex is on its own mesh
+    shots = [(M, P, 70, 1.0), (M, P, 66, 0.9), (MQ, Q, 50, 1.0), (M, P, 72, 1.1), (M, P, 40, 1.0)]
+    alone = [_alone(m, dict(p, nsteps=n), f) for m, p, n, f in shots]
+    out = {}
+    for cap in (4, 1):
+        S = sem.Server(block_steps=8, shots_per_group=cap)
+        ids = [S.submit(m, n, p["dt"], _sources(p, f), p["recs"]) for m, p, n, f in shots]
+        st = S.pump()
+        assert st["active"] == len(shots)
+        groups = [S.request(i)["group_shots"] for i in ids]
+        assert groups == ([3, 3, 1, 3, 1] if cap == 4 else [1] * 5)
+        S.drain()
+        infos = [S.request(i) for i in ids]
+        for i, inf in zip(ids, infos):
+            assert inf["state"] == "done" and inf["steps"] == S.requests[i][1]
+        if cap == 4:
+            # the 66-step request is copied out before its 72-step group ends
+            assert infos[1]["done"] <= infos[3]["done"]
+        out[cap] = [S.result(i) for i in ids]
+        S.close()
+    for k in range(len(shots)):
+        np.testing.assert_array_equal(out[4][k], alone[k])
+        np.testing.assert_array_equal(out[1][k], alone[k])
+    assert np.abs(alone[1]).max() > 0
+
+
+def test_cancel_inside_a_group():
+    """Cancelling one request of a multi-shot group leaves the others'
+    results unchanged; the group's memory goes when its last request ends."""
+    P = _problems()[1]
+    M = _mesh(P)
+    alone = _alone(M, dict(P, nsteps=64))
+    S = sem.Server(block_steps=8)
+    a = S.submit(M, 64, P["dt"], _sources(P), P["recs"])
+    b = S.submit(M, 64, P["dt"], _sources(P, 0.9), P["recs"])
+    st = S.pump()
+    assert S.request(a)["group_shots"] == 2 and st["active"] == 2
+    held = st["bytes"]
+    S.cancel(b)
+    st = S.pump()
+    assert st["active"] == 1 and st["bytes"] == held  # shared with a: kept until the group ends
+    S.drain()
+    assert S.request(b)["state"] == "cancelled"
+    np.testing.assert_array_equal(S.result(a), alone)
+    assert S.pump()["bytes"] == 0
+
+
 def _full_size_models():
     """One model per configuration-5 category at the top of its size range:
     800^2 acoustic quads, 1000^2 elastic quads, 96^3 elastic hexes."""
