@@ -14,6 +14,12 @@ node-component updates over the makespan), for the batched server
 (continuous batching, all active requests in the same launches) or the plain
 one-shot-per-launch server (one request at a time, FIFO).
 
+The plain server on a whole configuration-5 stream runs for tens of
+minutes; --plain-model gives its latencies from a FIFO schedule of measured
+per-model service times (set-up plus steps, each model timed alone on the
+plain server), and --plain-prefix N checks that model against the live plain
+server on the first N requests.
+
 python -m salvus_amd.serve [--requests N] [--rate R] [--scale S] ...
 """
 import argparse
@@ -134,6 +140,45 @@ def run(models, reqs, path=sem.PATH_FUSED, max_active=0, block_steps=32, hbm_bud
                 requests=len(reqs), dof_updates=dofs), results
 
 
+def plain_service_times(models, reqs, short=(100, 400)):
+    """Measured service time of every model on the plain one-shot-per-launch
+    server: one request alone at two step counts (admission, set-up, steps,
+    retirement), fitted as a + b * nsteps per model."""
+    fit = {}
+    for mi in sorted({r["model"] for r in reqs}):
+        m = models[mi]
+        r0 = next(r for r in reqs if r["model"] == mi)
+        t = []
+        for n in short:
+            S = sem.Server(path=sem.PATH_PLAIN, max_active=1)
+            t0 = S.clock()
+            i = S.submit(m["mesh"], n, m["dt"], _sources(m, r0), m["recs"], arrival=t0)
+            S.drain()
+            t.append(S.request(i)["done"] - t0)
+            S.close()
+        b = (t[1] - t[0]) / (short[1] - short[0])
+        fit[mi] = (t[0] - b * short[0], b)
+    return fit
+
+
+def fifo_schedule(models, reqs, fit):
+    """Latencies of a one-at-a-time FIFO server with the fitted service
+    times: start = max(arrival, previous finish), finish = start + a + b n."""
+    lat, dofs, t_free = [], 0, 0.0
+    for r in reqs:
+        a, b = fit[r["model"]]
+        start = max(r["arrival"], t_free)
+        t_free = start + a + b * r["nsteps"]
+        lat.append(t_free - r["arrival"])
+        m = models[r["model"]]["mesh"]
+        dofs += m.c * m.G * r["nsteps"]
+    lat = np.array(lat)
+    span = t_free - reqs[0]["arrival"]
+    return dict(p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                mean_s=float(lat.mean()), dof_updates_per_s=dofs / span, makespan_s=span,
+                requests=len(reqs), dof_updates=dofs)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--requests", type=int, default=2000)
@@ -143,6 +188,13 @@ def main():
     ap.add_argument("--models", type=int, default=4, help="models per category")
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--plain", action="store_true", help="also serve through the plain path")
+    ap.add_argument("--plain-model", action="store_true",
+                    help="plain FIFO server on the whole stream from measured per-model "
+                         "service times")
+    ap.add_argument("--plain-prefix", type=int, default=0,
+                    help="serve the first N requests live through the plain server and "
+                         "compare with the service-time model on the same N")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     lo, hi = (int(x) for x in args.steps.split(","))
     specs, reqs = draw_stream(args.requests, args.rate, args.seed, args.models, args.scale,
@@ -152,10 +204,26 @@ def main():
     def progress(tag):
         return lambda st: print("[serve %s] done %d, active %d, pending %d" %
                                 (tag, st["done"], st["active"], st["pending"]), flush=True)
-    out = {"batched": run(models, reqs, progress=progress("batched"))[0]}
+    out = {}
+    if not args.no_batched:
+        out["batched"] = run(models, reqs, progress=progress("batched"))[0]
+        print("[serve] batched " + json.dumps(out["batched"]), flush=True)
     if args.plain:
         out["plain"] = run(models, reqs, path=sem.PATH_PLAIN, max_active=1,
                            progress=progress("plain"))[0]
+    if args.plain_model or args.plain_prefix:
+        fit = plain_service_times(models, reqs)
+        out["plain_service_fit"] = {str(k): dict(setup_s=v[0], step_s=v[1]) for k, v in fit.items()}
+        if args.plain_model:
+            out["plain_model"] = fifo_schedule(models, reqs, fit)
+            print("[serve] plain model " + json.dumps(out["plain_model"]), flush=True)
+        if args.plain_prefix:
+            pre = reqs[:args.plain_prefix]
+            out["plain_prefix"] = dict(
+                measured=run(models, pre, path=sem.PATH_PLAIN, max_active=1,
+                             progress=progress("plain prefix"))[0],
+                model=fifo_schedule(models, pre, fit))
+            print("[serve] plain prefix " + json.dumps(out["plain_prefix"]), flush=True)
     out["models"] = [dict(dim=s["dim"], physics=s["physics"], n=s["n"]) for s in specs]
     print(json.dumps(out))
 
