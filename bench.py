@@ -260,6 +260,7 @@ def main():
                                     "(affine factors + vertex material); per node S (4 + 36 c)",
                      "tile_bytes_per_launch": tile_launch_bytes,
                      "tile_achieved_GBs": tile_achieved,
+                     "tile_frac": tile_achieved / HBM_PEAK_GBS,
                      "avg_launch_us": avg_launch_s * 1e6,
                      "launches_per_step": launches_per_step,
                      "kernel_ms_per_step": kms / timing_steps},
