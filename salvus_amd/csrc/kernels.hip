@@ -511,6 +511,18 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
   if (step >= G.nsteps) return;  // uniform across the block
   const gptr<float> gu = glb(G.u), gw = glb(G.w), gap = glb(G.ap), gacc = glb(G.acc);
   const gptr<uint32_t> gkc = glb(G.kc);
+  // descriptor fields the shot loop needs, read once: G sits in global memory,
+  // so a field read after the loop's global stores would be reloaded (the
+  // compiler cannot rule out aliasing), and waiting for that load would wait
+  // for every store before it
+  const int64_t GG = G.G;
+  const float gdt = G.dt;
+  const gptr<const float> gext = glb(G.ext), gstf = glb(G.stf), gload = glb(G.load);
+  const bool has_ext = G.ext != nullptr;
+  const gptr<const SrcDev> gsrc = glb(G.src);
+  const int nsrc = G.nsrc;
+  const bool q3 = (C >= 2) && (G.quirks & SEM_QUIRK_NEWMARK3D);  // {x, y, x}: x twice, z never
+  const bool wap = A.koff == A.ap_koff || step + 1 >= G.nsteps || G.ap_always;
 
   const int t = threadIdx.x;
   const int el = t / S::ND;
@@ -543,9 +555,9 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
     if (active) {
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        const int64_t idx = ((int64_t)s * C + c) * G.G + g;
+        const int64_t idx = ((int64_t)s * C + c) * GG + g;
         if (s == 0) su[c * S::ND + p] = gu[idx];
-        pacc[c] = first ? (G.ext ? glb(G.ext)[idx] : 0.f) : gacc[idx];
+        pacc[c] = first ? (has_ext ? gext[idx] : 0.f) : gacc[idx];
         pw[c] = fin ? gw[idx] : 0.f;
         pkc[c] = fin ? gkc[idx] : 0u;
       }
@@ -557,46 +569,45 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
       float nu[C];
       const bool more = s + 1 < Sg;
 #pragma unroll
-      for (int c = 0; c < C; ++c) nu[c] = more ? gu[((int64_t)(s + 1) * C + c) * G.G + g] : 0.f;
+      for (int c = 0; c < C; ++c) nu[c] = more ? gu[((int64_t)(s + 1) * C + c) * GG + g] : 0.f;
       float f[C];
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         float ku = QREF ? X.div_ref(sF + c * DIM * S::ND, M.Jinv) : X.div(sF + c * DIM * S::ND);
         f[c] = (word & kFlagDirichlet) ? 0.f : -ku;
       }
-      for (int k = 0; k < G.nsrc; ++k) {
-        const SrcDev sd = glb(G.src)[k];
+      for (int k = 0; k < nsrc; ++k) {
+        const SrcDev sd = gsrc[k];
         if (sd.elem == e && sd.shot == s) {
-          float R = glb(G.stf)[sd.stf_off + step];
+          float R = gstf[sd.stf_off + step];
 #pragma unroll
-          for (int c = 0; c < C; ++c) f[c] = fmaf(glb(G.load)[sd.load_off + c * S::ND + p], R, f[c]);
+          for (int c = 0; c < C; ++c) f[c] = fmaf(gload[sd.load_off + c * S::ND + p], R, f[c]);
         }
       }
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        const int64_t idx = ((int64_t)s * C + c) * G.G + g;
-        const float t = (first && !G.ext) ? f[c] : pacc[c] + f[c];
+        const int64_t idx = ((int64_t)s * C + c) * GG + g;
+        const float t = (first && !has_ext) ? f[c] : pacc[c] + f[c];
         if (!fin) gacc[idx] = t;
         else f[c] = t;
       }
       if (fin) {
         // Order2Newmark.cpp:68-115; with the field-list quirk every elastic
         // mesh advances x twice, y once and (3-D) z never
-        const bool q3 = (C >= 2) && (G.quirks & SEM_QUIRK_NEWMARK3D);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           if (q3 && c == 2) continue;  // c == 2 exists only in 3-D
-          const int64_t idx = ((int64_t)s * C + c) * G.G + g;
+          const int64_t idx = ((int64_t)s * C + c) * GG + g;
           float a = mi * f[c];
           float w = pw[c], kw = comp_v(pkc[c]);
           float u = su[c * S::ND + p], ku = comp_u(pkc[c]);
           const int reps = (q3 && c == 0) ? 2 : 1;
           for (int r = 0; r < reps; ++r) {  // w = v + dt/2 a_: w += dt a, u += dt w
-            kahan_add(w, kw, G.dt * a);
-            kahan_add(u, ku, G.dt * w);
+            kahan_add(w, kw, gdt * a);
+            kahan_add(u, ku, gdt * w);
           }
           gu[idx] = u; gw[idx] = w; gkc[idx] = pack_comp(ku, kw);
-          if (A.koff == A.ap_koff || step + 1 >= G.nsteps || G.ap_always) gap[idx] = a;  // read out only
+          if (wap) gap[idx] = a;  // read out only
         }
       }
       // own slot only: every thread's flux stage has read su (barrier above)
