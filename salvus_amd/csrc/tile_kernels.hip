@@ -236,6 +236,20 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
   const TileInfo ti = glb(G.tiles)[W.first];
   const int tid = threadIdx.x;
   const TileTables &TT = t_tab[N];
+  // descriptor fields, read once: G sits in global memory, so a field read
+  // after a global store would be reloaded (the compiler cannot rule out
+  // aliasing), and waiting for that load would wait for every store before it
+  const gptr<float> gu = glb(G.u);
+  const gptr<float> gw = glb(G.w);
+  const gptr<float> gap = glb(G.ap);
+  const gptr<uint32_t> gkc = glb(G.kc);
+  const gptr<float> gacc = glb(G.acc);
+  const gptr<const float> gmi = glb(G.mi);
+  const uint32_t GG = (uint32_t)G.G, Sg = (uint32_t)G.S;
+  const float gdt = G.dt;
+  const gptr<const float> gstf = glb(G.stf), gload = glb(G.load);
+  const gptr<const SrcDev> gsrc = glb(G.src);
+  const int nsrc = G.nsrc;
 
   // ---- once per tile: node words, element data, sources inside the tile,
   // and the lattice of shot 0 (every later shot's lattice is loaded during
@@ -265,6 +279,9 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
     }
   };
   {
+    // every load that does not need the words goes out with them: element
+    // data, the derivative table and the tile's sources; then the loads
+    // that do (inverse mass, shot 0's lattice)
     const gptr<const uint32_t> words =
         (T::NODE_ORDER ? glb(G.tile_words) : glb(G.tile_words_lat)) + ti.node_off;
     uint32_t wv[T::NIT];
@@ -278,30 +295,24 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
         else if (k < T::NTL) ssl[k] = (uint16_t)slot;
       }
     }
-    float uv[T::NIT][C];
-#pragma unroll
-    for (int r = 0; r < (kKeepMi ? T::KF : 0); ++r) {
-      const bool fin0 = wv[r] != kSlotEmpty && (wv[r] & kFlagLast) && !A.ku_only;
-      mik[r] = fin0 ? glb(G.mi)[wv[r] & kNodeMask] : 0.f;
-    }
-#pragma unroll
-    for (int r = 0; r < T::NIT; ++r) {
-      const bool ok = wv[r] != kSlotEmpty;
-      const uint32_t g = ok ? (wv[r] & kNodeMask) : 0u;
-#pragma unroll
-      for (int c = 0; c < C; ++c) uv[r][c] = ok ? glb(G.u)[c * (uint32_t)G.G + g] : 0.f;
-    }
-    for (int i = tid; i < n * n; i += T::BT) sD[i] = TT.D[i];
+    constexpr int NEI = (T::E * T::NED + T::BT - 1) / T::BT;
+    static_assert(n * n <= T::BT, "derivative table staged one entry per thread");
     const gptr<const float> te = glb(G.telem) + (int64_t)ti.elem_off * T::NED;
-    for (int i = tid; i < T::E * T::NED; i += T::BT) sE[i] = te[i];
+    float ev[NEI];
+#pragma unroll
+    for (int q = 0; q < NEI; ++q) {
+      const int i = tid + q * T::BT;
+      ev[q] = i < T::E * T::NED ? te[i] : 0.f;
+    }
+    const float dv = tid < n * n ? TT.D[tid] : 0.f;
     if (tid < 64) {  // ordered compaction by wave 0 (ballot + prefix count)
       int cnt = 0;
-      for (int j0 = 0; j0 < G.nsrc; j0 += 64) {
+      for (int j0 = 0; j0 < nsrc; j0 += 64) {
         const int j = j0 + tid;
         SrcDev sd{0, 0, 0, 0};
         bool in = false;
-        if (j < G.nsrc) {
-          sd = glb(G.src)[j];
+        if (j < nsrc) {
+          sd = gsrc[j];
           in = sd.elem >= ti.elem_off && sd.elem < ti.elem_off + T::E;
         }
         const unsigned long long mask = __ballot(in);
@@ -317,6 +328,25 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
         cnt += __popcll(mask);
       }
       if (tid == 0) sS[4 * kTileMaxSrc] = cnt < kTileMaxSrc ? cnt : kTileMaxSrc;
+    }
+    float uv[T::NIT][C];
+#pragma unroll
+    for (int r = 0; r < (kKeepMi ? T::KF : 0); ++r) {
+      const bool fin0 = wv[r] != kSlotEmpty && (wv[r] & kFlagLast) && !A.ku_only;
+      mik[r] = fin0 ? gmi[wv[r] & kNodeMask] : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < T::NIT; ++r) {
+      const bool ok = wv[r] != kSlotEmpty;
+      const uint32_t g = ok ? (wv[r] & kNodeMask) : 0u;
+#pragma unroll
+      for (int c = 0; c < C; ++c) uv[r][c] = ok ? gu[c * GG + g] : 0.f;
+    }
+    if (tid < n * n) sD[tid] = dv;
+#pragma unroll
+    for (int q = 0; q < NEI; ++q) {
+      const int i = tid + q * T::BT;
+      if (i < T::E * T::NED) sE[i] = ev[q];
     }
 #pragma unroll
     for (int r = 0; r < T::NIT; ++r) {
@@ -339,23 +369,10 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
   const float xa = TT.xi[ci], xb = DIM == 3 ? TT.xi[cj] : 0.f;
   const int pl = ci + (DIM == 3 ? n * cj : 0);  // element-local index of the column's node 0
 
-  const gptr<float> gu = glb(G.u);
-  const gptr<float> gw = glb(G.w);
-  const gptr<float> gap = glb(G.ap);
-  const gptr<uint32_t> gkc = glb(G.kc);
-  const gptr<float> gacc = glb(G.acc);
-  const gptr<const float> gmi = glb(G.mi);
-  const uint32_t GG = (uint32_t)G.G, Sg = (uint32_t)G.S;
   const bool q3 = (C >= 2) && (G.quirks & SEM_QUIRK_NEWMARK3D);  // {x, y, x}: x twice, z never
   // a_ is only read out (v = w - dt/2 a_, a_ itself): stored at the block's
   // last step and at the shot's last step, where a read-out can follow
   const bool wap = A.koff == A.ap_koff || step + 1 >= G.nsteps || G.ap_always;
-  // descriptor fields the shot loop needs, read once: G sits in global memory,
-  // so a field read after the loop's global stores would be reloaded (the
-  // compiler cannot rule out aliasing), and waiting for that load would wait
-  // for every store before it
-  const float gdt = G.dt;
-  const gptr<const float> gstf = glb(G.stf), gload = glb(G.load);
   const gptr<const float> gext = glb(G.ext);  // coupled shots: external load
   const bool has_ext = G.ext != nullptr;
 
