@@ -525,6 +525,12 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
   const bool wap = A.koff == A.ap_koff || step + 1 >= G.nsteps || G.ap_always;
 
   const int t = threadIdx.x;
+  // the group's first sources staged in LDS once per block (read after the
+  // shot loop's barriers), so that a node's source scan is no chain of
+  // dependent global loads
+  constexpr int kMaxStaged = 64;
+  __shared__ SrcDev ssrc[kMaxStaged];
+  for (int k = t; k < nsrc && k < kMaxStaged; k += blockDim.x) ssrc[k] = gsrc[k];
   const int el = t / S::ND;
   const int p = t - el * S::ND;
   const bool active = el < W.count;
@@ -553,13 +559,21 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
     float pacc[C], pw[C];
     uint32_t pkc[C];
     if (active) {
+      // every load first, the LDS stores of shot 0's u after them (a store
+      // between the loads would wait for its own load before the next one
+      // is issued)
+      float u0[C];
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         const int64_t idx = ((int64_t)s * C + c) * GG + g;
-        if (s == 0) su[c * S::ND + p] = gu[idx];
+        u0[c] = s == 0 ? gu[idx] : 0.f;
         pacc[c] = first ? (has_ext ? gext[idx] : 0.f) : gacc[idx];
         pw[c] = fin ? gw[idx] : 0.f;
         pkc[c] = fin ? gkc[idx] : 0u;
+      }
+      if (s == 0) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) su[c * S::ND + p] = u0[c];
       }
     }
     __syncthreads();
@@ -576,14 +590,16 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
         float ku = QREF ? X.div_ref(sF + c * DIM * S::ND, M.Jinv) : X.div(sF + c * DIM * S::ND);
         f[c] = (word & kFlagDirichlet) ? 0.f : -ku;
       }
-      for (int k = 0; k < nsrc; ++k) {
-        const SrcDev sd = gsrc[k];
+      auto add_source = [&](const SrcDev &sd) {
         if (sd.elem == e && sd.shot == s) {
           float R = gstf[sd.stf_off + step];
 #pragma unroll
           for (int c = 0; c < C; ++c) f[c] = fmaf(gload[sd.load_off + c * S::ND + p], R, f[c]);
         }
-      }
+      };
+      const int nsl = nsrc < kMaxStaged ? nsrc : kMaxStaged;
+      for (int k = 0; k < nsl; ++k) add_source(ssrc[k]);           // LDS
+      for (int k = kMaxStaged; k < nsrc; ++k) add_source(gsrc[k]);  // beyond the staged ones
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         const int64_t idx = ((int64_t)s * C + c) * GG + g;
