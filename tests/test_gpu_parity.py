@@ -316,3 +316,31 @@ def test_diagonal_geometry_variant_identical(dim, physics, monkeypatch):
         out[general] = (seis, u)
     np.testing.assert_array_equal(out["0"][0], out["1"][0])
     np.testing.assert_array_equal(out["0"][1], out["1"][1])
+
+
+@pytest.mark.parametrize("dim,order", [(2, o) for o in range(1, 11)] + [(3, o) for o in range(1, 8)])
+@pytest.mark.parametrize("physics", [0, 1])
+def test_every_compiled_order(dim, order, physics):
+    """Every order the element kernel and the plain path are compiled for
+    (2-D 1-10, 3-D 1-7): the stiffness operator on a distorted mesh (element
+    kernel, pointwise geometry) and on an affine one (tile kernel where one
+    is compiled, else the element kernel), both paths, and the inverse mass,
+    against the oracle (measured up to 3e-7 relative L2 at every order)."""
+    tol = TOL_OP
+    nel = 3 if dim == 3 else 4
+    for distorted in (True, False):
+        P = pb.make(dim, physics, nel, order=order, seed=order + 20, distorted=distorted)
+        M = mesh_for(P)
+        S = oracle_for(P)
+        mi = M.inverse_mass().astype(np.float64)
+        assert np.max(np.abs(mi - S.inverse_mass()) / S.inverse_mass()) < 1e-6
+        rng = np.random.default_rng(order)
+        xyz = S.node_coords()
+        u = np.stack([np.sin(0.05 * xyz @ rng.normal(size=dim)) + 0.1 * rng.normal(size=S.G)
+                      for _ in range(S.c)])
+        ref = S.apply_stiffness(u)
+        for path in (sem.PATH_FUSED, sem.PATH_PLAIN):
+            err = pb.rel_l2(M.apply_stiffness(u, path).astype(np.float64), ref)
+            print("dim %d order %d physics %d distorted %d path %d tiles %s: %.2e"
+                  % (dim, order, physics, distorted, path, M.tile_shape, err), flush=True)
+            assert err < tol, (distorted, path, err)
