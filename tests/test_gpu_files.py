@@ -129,7 +129,8 @@ def test_movie_every_step(tmp_path):
 def test_exodus_mesh_and_model_run(tmp_path, dim, physics, distorted):
     """A mesh and model written as an Exodus II file, read back through
     exodus.read_mesh / read_model and model.vertex_parameters, run exactly
-    as the arrays they were written from."""
+    as the arrays they were written from, and within the run tolerance of
+    the FP64 oracle given the arrays read back."""
     from salvus_amd import exodus, model
     from tests.test_exodus import write_exodus
     P = pb.make(dim, physics, 5 if dim == 3 else 10, order=4, seed=90 + dim, distorted=distorted,
@@ -147,4 +148,14 @@ def test_exodus_mesh_and_model_run(tmp_path, dim, physics, distorted):
     M0 = sem.Mesh(dim, 4, physics, cells, P["coords"], P["par"])
     M1 = sem.Mesh(d, 4, physics, rc, rxyz, par)
     assert (M1.ntiles > 0) == (not distorted)
-    np.testing.assert_array_equal(run(M1, P), run(M0, P))
+    got = run(M1, P)
+    np.testing.assert_array_equal(got, run(M0, P))
+    from oracle import pyoracle as po
+    S = po.Sim(d, 4, physics, rc, rxyz, par)
+    s = P["src"]
+    S.add_source(s["loc"], s["amp"], s["f0"], s["t0"], s["dir"])
+    for r in P["recs"]:
+        S.add_receiver(r)
+    ref = S.run(P["dt"], P["nsteps"])
+    assert np.abs(ref).max() > 0
+    assert pb.rel_l2(got, ref) < 1e-5
