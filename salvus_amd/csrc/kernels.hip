@@ -552,28 +552,26 @@ __global__ void __launch_bounds__(512) k_fused(FusedArgs A) {
   const int Sg = G.S;
   const bool first = (word & kFlagFirst) != 0;
   const bool fin = (word & kFlagLast) && !A.ku_only;
+  // shot 0's u gathered before the shot loop (every later shot's u is loaded
+  // while the previous shot integrates and finishes)
+  if (active) {
+    float u0[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) u0[c] = gu[(int64_t)c * GG + g];
+#pragma unroll
+    for (int c = 0; c < C; ++c) su[c * S::ND + p] = u0[c];
+  }
   for (int s = 0; s < Sg; ++s) {
-    // gather (shot 0 only: every later shot's u is loaded while the previous
-    // shot integrates and finishes, and staged after this thread's own
-    // finish), and this node's finish inputs loaded ahead of the element work
+    // this node's finish inputs, loaded ahead of the element work
     float pacc[C], pw[C];
     uint32_t pkc[C];
     if (active) {
-      // every load first, the LDS stores of shot 0's u after them (a store
-      // between the loads would wait for its own load before the next one
-      // is issued)
-      float u0[C];
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         const int64_t idx = ((int64_t)s * C + c) * GG + g;
-        u0[c] = s == 0 ? gu[idx] : 0.f;
         pacc[c] = first ? (has_ext ? gext[idx] : 0.f) : gacc[idx];
         pw[c] = fin ? gw[idx] : 0.f;
         pkc[c] = fin ? gkc[idx] : 0u;
-      }
-      if (s == 0) {
-#pragma unroll
-        for (int c = 0; c < C; ++c) su[c * S::ND + p] = u0[c];
       }
     }
     __syncthreads();
