@@ -122,12 +122,13 @@ def run(models, reqs, path=sem.PATH_FUSED, max_active=0, block_steps=32, hbm_bud
         ids.append(S.submit(m["mesh"], r["nsteps"], m["dt"], _sources(m, r), m["recs"],
                             arrival=t_start + r["arrival"]))
     S.drain(progress=progress)
-    lat, dofs, t_last = [], 0, t_start
+    lat, dofs, t_last, grouped = [], 0, t_start, []
     for r, i in zip(reqs, ids):
         info = S.request(i)
         if info["state"] != "done":
             raise sem.SemError("request %d ended in state %s" % (i, info["state"]))
         lat.append(info["done"] - info["arrival"])
+        grouped.append(info["group_shots"])
         m = models[r["model"]]["mesh"]
         dofs += m.c * m.G * r["nsteps"]
         t_last = max(t_last, info["done"])
@@ -135,9 +136,12 @@ def run(models, reqs, path=sem.PATH_FUSED, max_active=0, block_steps=32, hbm_bud
     S.close()
     lat = np.array(lat)
     span = t_last - (t_start + reqs[0]["arrival"])
+    grouped = np.array(grouped)
     return dict(p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
                 mean_s=float(lat.mean()), dof_updates_per_s=dofs / span, makespan_s=span,
-                requests=len(reqs), dof_updates=dofs), results
+                requests=len(reqs), dof_updates=dofs,
+                in_multi_shot_groups=int((grouped > 1).sum()),
+                mean_group_shots=float(grouped.mean())), results
 
 
 def plain_service_times(models, reqs, short=(100, 400)):
