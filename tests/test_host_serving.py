@@ -59,3 +59,22 @@ def test_byte_models_on_a_hand_case():
     assert bench.tile_bytes_per_step(M, 1, 2, 0) == 16 + 28 + 16 + 4 * 24
     # S shots scale the per-shot parts only
     assert bench.tile_bytes_per_step(M, 3, 2, 0) == 16 + 28 + 3 * (16 + 4 * 24)
+
+
+def test_fifo_schedule_of_service_times():
+    """The plain server's FIFO schedule: a request starts at its arrival or
+    when the one before it finishes, whichever is later, and takes the
+    model's set-up plus steps times the per-step time."""
+    M = types.SimpleNamespace(c=1, G=100)
+    models = [dict(mesh=M)]
+    fit = {0: (1.0, 0.01)}  # 1 s set-up, 10 ms per step
+    reqs = [dict(model=0, arrival=0.0, nsteps=100),   # 0 -> 2
+            dict(model=0, arrival=0.5, nsteps=200),   # 2 -> 5
+            dict(model=0, arrival=10.0, nsteps=100)]  # 10 -> 12 (idle before)
+    out = serve.fifo_schedule(models, reqs, fit)
+    lat = np.array([2.0, 4.5, 2.0])
+    assert out["makespan_s"] == 12.0
+    assert out["dof_updates"] == 100 * 400
+    assert abs(out["mean_s"] - lat.mean()) < 1e-12
+    assert abs(out["p50_s"] - np.percentile(lat, 50)) < 1e-12
+    assert abs(out["dof_updates_per_s"] - 100 * 400 / 12.0) < 1e-9
