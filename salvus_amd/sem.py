@@ -329,6 +329,29 @@ class Mesh:
         return int(v[0])
 
 
+def split_coupled(cells, coords, fluid, vertex_par):
+    """The fluid and the solid part of a mixed quad mesh (the reference's
+    single mesh whose elements are Scalar or Elastic2D, Mesh.cpp:480-521), as
+    the inputs of two meshes: returns ((cells, coords, vertex_par) of the
+    fluid cells, the same of the solid cells), each with its vertices
+    renumbered in their original order.  `vertex_par` is a pair (fluid
+    cells x 4 x 1 VP, solid cells x 4 x 5) or None."""
+    cells = np.asarray(cells)
+    coords = np.asarray(coords, dtype=np.float64)
+    fluid = np.asarray(fluid, dtype=bool)
+    if cells.ndim != 2 or cells.shape[1] != 4 or fluid.shape != (len(cells),):
+        raise SemError("split_coupled needs quad cells (E x 4) and one flag per cell")
+    out = []
+    for k, sel in enumerate((fluid, ~fluid)):
+        sub = cells[sel]
+        used = np.unique(sub)
+        remap = np.full(len(coords), -1, dtype=np.int64)
+        remap[used] = np.arange(len(used))
+        par = None if vertex_par is None else vertex_par[k]
+        out.append((remap[sub].astype(np.int32), coords[used], par))
+    return out[0], out[1]
+
+
 class Coupling:
     """2-D fluid-solid coupling of an acoustic and an elastic mesh cut from
     one quad mesh (the coupled elements of Mesh.cpp:480-521 with the edge

@@ -47,11 +47,9 @@ def _problem(interface, distorted):
 
 
 def _part(cells, coords, idx):
-    sub = cells[idx]
-    used = np.unique(sub)
-    remap = np.full(len(coords), -1)
-    remap[used] = np.arange(len(used))
-    return remap[sub].astype(np.int32), coords[used]
+    fluid = np.zeros(len(cells), dtype=bool)
+    fluid[idx] = True
+    return sem.split_coupled(cells, coords, fluid, None)[0][:2]
 
 
 @pytest.mark.parametrize("path", [sem.PATH_FUSED, sem.PATH_PLAIN])

@@ -5,6 +5,7 @@ count what they state on a hand-checked case."""
 import types
 
 import numpy as np
+import pytest
 
 import bench
 from salvus_amd import sem, serve
@@ -78,3 +79,20 @@ def test_fifo_schedule_of_service_times():
     assert abs(out["mean_s"] - lat.mean()) < 1e-12
     assert abs(out["p50_s"] - np.percentile(lat, 50)) < 1e-12
     assert abs(out["dof_updates_per_s"] - 100 * 400 / 12.0) < 1e-9
+
+
+def test_split_coupled_parts():
+    """The fluid and solid parts of a mixed quad mesh keep their cells'
+    vertex coordinates and order, share exactly the interface vertices, and
+    match the coupled oracle's own split."""
+    from salvus_amd import meshgen
+    cells, coords = meshgen.box_quad(6, 4, 60.0, 40.0)
+    fluid = coords[cells].mean(axis=1)[:, 0] < 30.0
+    (cf, xf, _), (cs, xs, _) = sem.split_coupled(cells, coords, fluid, None)
+    np.testing.assert_array_equal(xf[cf], coords[cells[fluid]])
+    np.testing.assert_array_equal(xs[cs], coords[cells[~fluid]])
+    assert cf.max() + 1 == len(xf) and cs.max() + 1 == len(xs)
+    shared = {tuple(p) for p in xf} & {tuple(p) for p in xs}
+    assert len(shared) == 5 and all(p[0] == 30.0 for p in shared)  # the line x = 30
+    with pytest.raises(sem.SemError):
+        sem.split_coupled(cells, coords, fluid[:-1], None)
