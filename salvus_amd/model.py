@@ -16,6 +16,7 @@ Elastic2D.cpp:29-36, Elastic3D.cpp:33-40):
   Elastic3D        RHO, VPV, VPH, VSV, VSH, ETA
 
 `vertex_parameters` returns the E x nv x npar array that `sem.Mesh` takes;
+`element_fluid` and `coupled_parameters` serve a mixed fluid/elastic model;
 `exodus.read_model` supplies the model's element centres and elemental
 variables from an Exodus file.
 """
@@ -64,3 +65,45 @@ def vertex_parameters(dim, physics, cells, coords, model_centres, variables):
         for v in range(nv):
             out[:, v, q] = np.asarray(variables["%s_%d" % (name, v)], dtype=np.float64)[nearest]
     return out
+
+
+def element_fluid(dim, cells, coords, model_centres, variables):
+    """Per element, True where the model marks it fluid and False where it
+    marks it elastic (ExodusModel.cpp:473-518): the elemental variable
+    "fluid" of the model element nearest the element's centre, truncated to
+    an integer, 1 fluid, 0 elastic; any other value, or no "fluid" variable,
+    is an error.  A hexahedral mesh that mixes the two is refused, as the
+    reference builds no 3-D fluid-solid coupled element (Element.cpp:60-112)."""
+    if "fluid" not in variables:
+        raise KeyError("Fluid flag not found in mesh. An elemental variable "
+                       "'fluid' (1 fluid, 0 elastic) must mark every element.")
+    cells = np.asarray(cells)
+    _, nearest = cKDTree(np.asarray(model_centres, dtype=np.float64)).query(
+        element_centres(cells, coords))
+    flag = np.trunc(np.asarray(variables["fluid"], dtype=np.float64)[nearest])
+    bad = (flag != 0) & (flag != 1)
+    if bad.any():
+        raise ValueError("element %d: unknown physics flag %g (0 elastic, 1 fluid)"
+                         % (int(np.flatnonzero(bad)[0]), flag[bad][0]))
+    fluid = flag == 1
+    if dim == 3 and fluid.any() and not fluid.all():
+        raise ValueError("a hexahedral mesh with both fluid and elastic elements "
+                         "has no coupled element to build")
+    return fluid
+
+
+def coupled_parameters(cells, coords, model_centres, variables):
+    """Everything a 2-D fluid-solid problem reads from the model: the fluid
+    flags (element_fluid), the fluid cells' vertex VP (E_f x 4 x 1), the
+    solid cells' vertex RHO, C11, C13, C33, C55 (E_s x 4 x 5) and the fluid
+    cells' vertex RHO (E_f x 4), whose mean per cell is the density rho_0 the
+    solid side of the interface reads (AcousticToElastic2D.cpp:33-41)."""
+    cells = np.asarray(cells)
+    fluid = element_fluid(2, cells, coords, model_centres, variables)
+    par_f = vertex_parameters(2, 0, cells[fluid], coords, model_centres, variables)
+    par_s = vertex_parameters(2, 1, cells[~fluid], coords, model_centres, variables)
+    _, nearest = cKDTree(np.asarray(model_centres, dtype=np.float64)).query(
+        element_centres(cells[fluid], coords))
+    rho = np.stack([np.asarray(variables["RHO_%d" % v], dtype=np.float64)[nearest]
+                    for v in range(cells.shape[1])], axis=1)
+    return fluid, par_f, par_s, rho

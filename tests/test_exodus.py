@@ -143,3 +143,56 @@ def test_errors(tmp_path):
     plain = tmp_path / "novars.e"
     write_exodus(plain, coords, [(cells + 1, "QUAD4")])
     assert exodus.read_model(plain)[2] == {}
+
+
+def _mixed_model_file(path, cells, coords, fluid):
+    """A quad model file with a per-element fluid flag, fluid VP and RHO and
+    solid RHO, C11, C13, C33, C55 (zeros where a family does not apply)."""
+    E = len(cells)
+    idx = np.arange(E, dtype=float)
+    ev = {"fluid": fluid.astype(float)}
+    for v in range(4):
+        ev["VP_%d" % v] = np.where(fluid, 1500.0 + idx + 0.25 * v, 0.0)
+        ev["RHO_%d" % v] = np.where(fluid, 1000.0 + 3 * idx + v, 2500.0 + v)
+        for k, name in enumerate(("C11", "C13", "C33", "C55")):
+            ev["%s_%d" % (name, v)] = np.where(fluid, 0.0, 1e10 * (k + 1) + 1e6 * idx + v)
+    write_exodus(path, coords, [(cells + 1, "QUAD4")], elem_vars=ev)
+    return ev
+
+
+def test_mixed_model_flags_and_coupled_parameters(tmp_path):
+    """ExodusModel::getElementType (ExodusModel.cpp:473-518): the "fluid"
+    variable of the nearest model element, 1 fluid, 0 elastic, other values
+    and a missing flag errors; the coupled inputs (fluid VP, solid RHO and
+    C-tensor, fluid vertex RHO for rho_0) come back per split part and feed
+    split_coupled."""
+    cells, coords = meshgen.box_quad(6, 4, 60.0, 40.0)
+    cent = model.element_centres(cells, coords)
+    fluid = (cent[:, 0] < 30.0) | (cent[:, 1] > 30.0)  # an L-shaped fluid
+    path = tmp_path / "mixed.e"
+    ev = _mixed_model_file(path, cells, coords, fluid)
+    dim, centres, variables = exodus.read_model(path)
+    got, par_f, par_s, rho = model.coupled_parameters(cells, coords, centres, variables)
+    np.testing.assert_array_equal(got, fluid)
+    for v in range(4):
+        np.testing.assert_array_equal(par_f[:, v, 0], ev["VP_%d" % v][fluid])
+        np.testing.assert_array_equal(rho[:, v], ev["RHO_%d" % v][fluid])
+        for q, name in enumerate(("RHO", "C11", "C13", "C33", "C55")):
+            np.testing.assert_array_equal(par_s[:, v, q], ev["%s_%d" % (name, v)][~fluid])
+    (cf, xf, pf), (cs, xs, ps) = sem.split_coupled(cells, coords, got, (par_f, par_s))
+    np.testing.assert_array_equal(xf[cf], coords[cells[fluid]])
+    np.testing.assert_array_equal(xs[cs], coords[cells[~fluid]])
+    assert pf is par_f and ps is par_s
+    # a flag read as 1.7 truncates to fluid; 2 is no physics; no flag at all
+    v2 = dict(variables, fluid=np.where(fluid, 1.7, 0.2))
+    np.testing.assert_array_equal(model.element_fluid(2, cells, coords, centres, v2), fluid)
+    with pytest.raises(ValueError, match="unknown physics"):
+        model.element_fluid(2, cells, coords, centres, dict(variables, fluid=np.full(len(cells), 2.0)))
+    with pytest.raises(KeyError, match="Fluid flag"):
+        model.element_fluid(2, cells, coords, centres, {k: a for k, a in variables.items() if k != "fluid"})
+    # hexahedra: all fluid or all elastic, never both
+    hc, hx = meshgen.box_hex(2, 1, 1, 20.0, 10.0, 10.0)
+    hcent = model.element_centres(hc, hx)
+    assert model.element_fluid(3, hc, hx, hcent, {"fluid": np.ones(2)}).all()
+    with pytest.raises(ValueError, match="hexahedral"):
+        model.element_fluid(3, hc, hx, hcent, {"fluid": np.array([1.0, 0.0])})

@@ -137,3 +137,59 @@ def test_coupling_errors():
         B.couple(cpl, kf, ks2)  # already coupled
     B.close()
     cpl.close()
+
+
+def test_mixed_exodus_model_coupled_run(tmp_path):
+    """One quad Exodus file whose elemental "fluid" flag marks a staircase
+    fluid region (ExodusModel.cpp:473-518), read through exodus.read_mesh /
+    read_model, flagged and served by model.coupled_parameters, cut by
+    split_coupled and coupled: within the run tolerance of the FP64 coupled
+    oracle given the same arrays, on the tile path."""
+    from salvus_amd import exodus, model
+    from tests.test_exodus import write_exodus
+    cells, coords, fluid = _problem("staircase", False)
+    par_f, rho_f, par_s = _materials(cells, fluid)
+    E = len(cells)
+    ev = {"fluid": fluid.astype(float)}
+    fi, si = np.flatnonzero(fluid), np.flatnonzero(~fluid)
+    for v in range(4):
+        vp = np.zeros(E)
+        vp[fi] = par_f[:, v, 0] + 10.0 * (fi % 5)  # VP varies per cell
+        ev["VP_%d" % v] = vp
+        for q, name in enumerate(("RHO", "C11", "C13", "C33", "C55")):
+            a = np.zeros(E)
+            a[si] = par_s[:, v, q]
+            if name == "RHO":
+                a[fi] = rho_f[:, v]
+            ev["%s_%d" % (name, v)] = a
+    path = tmp_path / "mixed.e"
+    write_exodus(path, coords, [(cells + 1, "QUAD4")], elem_vars=ev)
+    d, rc, rxyz, _ = exodus.read_mesh(path)
+    _, centres, variables = exodus.read_model(path)
+    flag, pf, ps, rho = model.coupled_parameters(rc, rxyz, centres, variables)
+    np.testing.assert_array_equal(flag, fluid)
+    np.testing.assert_array_equal(rho, rho_f)
+    (cf, xf, pf), (cs, xs, ps) = sem.split_coupled(rc, rxyz, flag, (pf, ps))
+    mf = sem.Mesh(2, 4, sem.ACOUSTIC, cf, xf, pf)
+    ms = sem.Mesh(2, 4, sem.ELASTIC, cs, xs, ps)
+    cpl = sem.Coupling(mf, ms, rho)
+    nsteps, dt = 400, 0.3 * H * 0.17 / 3000.0
+    f0 = 1500.0 / (8 * H)
+    C = cp.CoupledSim(4, rc, rxyz, flag, pf, rho, ps)
+    C.add_source("f", np.array([12.0, 21.0]), 1.0, f0, 1.2 / f0, None)
+    ref = C.run(dt, nsteps)
+    assert np.abs(ref["s"]).max() > 0  # the wave crossed into the solid
+    B = sem.Batch()
+    kf = B.add_shot(mf, nsteps, dt)
+    ks = B.add_shot(ms, nsteps, dt)
+    B.add_source(kf, [12.0, 21.0], 1.0, f0, 1.2 / f0)
+    B.couple(cpl, kf, ks)
+    B.prepare(sem.PATH_FUSED)
+    B.run(nsteps)
+    for key, k in (("f", kf), ("s", ks)):
+        got = B.get_field(k, "u").reshape(ref[key].shape)
+        err = np.abs(got - ref[key]).max() / np.abs(ref[key]).max()
+        print("exodus mixed %s: max rel err %.2e" % (key, err), flush=True)
+        assert err < TOL, (key, err)
+    B.close()
+    cpl.close()
