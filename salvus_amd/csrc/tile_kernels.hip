@@ -109,16 +109,39 @@ struct TileTables {
 };
 __constant__ TileTables t_tab[kTileMaxN + 1];
 
-template <int DIM, int N, int PHYS, int TX, int TY, int TZ, bool HO = false>
-struct TS {
+// Padding of the LDS lattice of the order-4 3-D elastic tiles (extra words
+// per lattice row and per plane), chosen by counting the bank conflicts of
+// the column phase's LDS addresses (32 banks of 4 bytes per half-wave, and 64
+// per wave): unpadded, most x-derivative and node-column reads of a wave hit
+// one bank twice, because rows 17 words apart and elements 4 words apart
+// land on the same banks.
+constexpr int lattice_pad_row(int tx) { return tx == 4 ? 4 : tx == 2 ? 2 : 0; }
+constexpr int lattice_pad_plane(int tx, int ty) { return tx == 4 && ty == 2 ? 7 : tx == 2 && ty == 2 ? 5 : 0; }
+// smallest v >= a with v = r (mod m)
+constexpr int round_to_residue(int a, int r, int m) { return a + (((r - a) % m) + m) % m; }
+
+// LDS layout and thread shape of one tile kernel.  PADDED spreads the column
+// phase's LDS accesses over the banks: padded lattice rows and planes (the
+// lattice slot of node (x, y, z) is x + LXP y + LXYP z), and per-element
+// strides of the lateral fluxes and the element results equal to NCOL modulo
+// 32 and 64, so that column thread t writes and reads bank (t + const) mod 32.
+// The slots in the tile tables are unpadded (x + LX y + LXY z).
+template <int DIM, int N, int PHYS, int TX, int TY, int TZ, bool HO, bool PADDED>
+struct TSL {
   static constexpr int n = N + 1;
   static constexpr int ND = DIM == 2 ? n * n : n * n * n;  // nodes per element
   static constexpr int NCOL = ND / n;                       // columns per element
   static constexpr int C = PHYS == SEM_ACOUSTIC ? 1 : DIM;
   static constexpr int NL = DIM - 1;                        // lateral axes
   static constexpr int LX = TX * N + 1, LY = TY * N + 1, LZ = DIM == 3 ? TZ * N + 1 : 1;
-  static constexpr int LXY = LX * LY, NTL = LXY * LZ;       // lattice slots
-  static constexpr int CS = DIM == 3 ? LXY : LX;            // lattice stride of the column axis
+  static constexpr int LXY = LX * LY, NTL = LXY * LZ;       // lattice slots (tile words)
+  static constexpr bool PAD = PADDED && DIM == 3 && C == 3 && N == 4 && !HO;
+  static constexpr int LXP = PAD ? LX + lattice_pad_row(TX) : LX;
+  static constexpr int LXYP = PAD ? LXP * LY + lattice_pad_plane(TX, TY) : LXY;
+  static constexpr int NTLP = DIM == 3 ? LXYP * LZ : LXP * LY;  // LDS lattice words
+  static constexpr int CS = DIM == 3 ? LXYP : LXP;              // lattice stride of the column axis
+  static constexpr int SFE = PAD ? round_to_residue(C * NL * ND, NCOL % 32, 32) : C * NL * ND;
+  static constexpr int SKE = PAD ? round_to_residue(C * ND, NCOL % 64, 64) : C * ND;
   static constexpr int ES = DIM == 3 ? n * n : n;           // element-local stride of the column axis
   static constexpr int E = TX * TY * (DIM == 3 ? TZ : 1);
   // column chunks of at most 512 threads, all chunks equally full; the
@@ -145,14 +168,14 @@ struct TS {
   // last chunk's results overwrite its own lateral fluxes once every column
   // has read them (one more barrier, less LDS): taken with a single chunk,
   // and with several where it fits more workgroups in a CU's 160 KiB.
-  static constexpr int O_SW = C * NTL;
-  static constexpr int LDS_REST = O_SW + NTL + E * NED + 4 * kTileMaxSrc + 1 + n * n;
-  static constexpr int LDS_SEP = LDS_REST + EC * C * NL * ND + E * C * ND;
-  static constexpr int LDS_ALIAS = LDS_REST + EC * C * NL * ND + (NCH - 1) * EC * C * ND;
-  static constexpr bool ALIAS = NCH == 1 || 40960 / LDS_ALIAS > 40960 / LDS_SEP;
-  static constexpr int O_SF = ALIAS ? O_SW + NTL + (NCH - 1) * EC * C * ND : O_SW + NTL;
-  static constexpr int O_SK = ALIAS ? O_SW + NTL : O_SF + EC * C * NL * ND;
-  static constexpr int O_SE = ALIAS ? O_SF + EC * C * NL * ND : O_SK + E * C * ND;
+  static constexpr int O_SW = C * NTLP;
+  static constexpr int LDS_REST = O_SW + NTLP + E * NED + 4 * kTileMaxSrc + 1 + n * n;
+  static constexpr int LDS_SEP = LDS_REST + EC * SFE + E * SKE;
+  static constexpr int LDS_ALIAS = LDS_REST + EC * SFE + (NCH - 1) * EC * SKE;
+  static constexpr bool ALIAS = (NCH == 1 && EC * SKE <= EC * SFE) || 40960 / LDS_ALIAS > 40960 / LDS_SEP;
+  static constexpr int O_SF = ALIAS ? O_SW + NTLP + (NCH - 1) * EC * SKE : O_SW + NTLP;
+  static constexpr int O_SK = ALIAS ? O_SW + NTLP : O_SF + EC * SFE;
+  static constexpr int O_SE = ALIAS ? O_SF + EC * SFE : O_SK + E * SKE;
   // Register budget: 4 waves per SIMD (<= 128 VGPRs) with the finish
   // inputs of every slot loaded ahead (up to KFMAX); the high-occupancy
   // variant HO (acoustic tiles) takes 6 (<= 80 VGPRs, a third workgroup per
@@ -171,7 +194,22 @@ struct TS {
   static constexpr bool SLOT_LDS = NODE_ORDER && C == 3 && E <= 8;
   static constexpr int O_SL = O_SD + n * n;
   static constexpr int LDS = O_SL + (SLOT_LDS ? (NTL + 1) / 2 : 0);
+  // padded LDS lattice slot of the unpadded slot l
+  static __device__ __forceinline__ int lds_slot(int l) {
+    if constexpr (!PAD) {
+      return l;
+    } else {
+      const int z = l / LXY, r = l - z * LXY, y = r / LX;
+      return (r - y * LX) + LXP * y + LXYP * z;
+    }
+  }
 };
+
+// The padded layout where it leaves at least as many workgroups per CU.
+template <int DIM, int N, int PHYS, int TX, int TY, int TZ, bool HO = false>
+struct TS : TSL<DIM, N, PHYS, TX, TY, TZ, HO,
+                40960 / TSL<DIM, N, PHYS, TX, TY, TZ, HO, true>::LDS >=
+                    40960 / TSL<DIM, N, PHYS, TX, TY, TZ, HO, false>::LDS> {};
 
 // a + b never fused with a product feeding it (per-operation contract flag)
 __device__ __forceinline__ float add_unfused(float a, float b) {
@@ -271,7 +309,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
   auto sl = [&](int r) {
     const int k = tid + r * T::BT;
     if constexpr (!T::NODE_ORDER) {
-      return k < T::NTL ? k : -1;
+      return k < T::NTL ? T::lds_slot(k) : -1;
     } else if constexpr (kSlotRegs) {
       return slr[r];
     } else {
@@ -290,7 +328,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
       const int k = tid + r * T::BT;
       wv[r] = k < T::NTL ? words[k] : kSlotEmpty;
       if constexpr (T::NODE_ORDER) {
-        const int slot = k < T::NTL ? (int)slots[k] : -1;
+        const int slot = k < T::NTL ? T::lds_slot((int)slots[k]) : -1;
         if constexpr (kSlotRegs) slr[r] = slot;
         else if (k < T::NTL) ssl[k] = (uint16_t)slot;
       }
@@ -354,7 +392,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
       if (l >= 0) {
         sw[l] = wv[r];
 #pragma unroll
-        for (int c = 0; c < C; ++c) su[c * T::NTL + l] = uv[r][c];
+        for (int c = 0; c < C; ++c) su[c * T::NTLP + l] = uv[r][c];
       }
     }
   }
@@ -390,10 +428,10 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
       if (lane && e < T::E) {
         const int ea = e % TX, eb = (e / TX) % TY, ez = DIM == 3 ? e / (TX * TY) : 0;
         act = ea < ti.ex && eb < ti.ey && (DIM == 2 || ez < ti.ez);
-        ebase = ea * N + T::LX * (eb * N) + (DIM == 3 ? T::LXY * (ez * N) : 0);
+        ebase = ea * N + T::LXP * (eb * N) + (DIM == 3 ? T::LXYP * (ez * N) : 0);
       }
-      const int cb = ebase + ci + (DIM == 3 ? T::LX * cj : 0);  // lattice slot of node 0
-      float *myF = sF + ec * (C * NL * ND);
+      const int cb = ebase + ci + (DIM == 3 ? T::LXP * cj : 0);  // lattice slot of node 0
+      float *myF = sF + ec * T::SFE;
       float Fz[C][n];
       if (act) {
         const float *ed = sE + e * T::NED;
@@ -407,7 +445,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
 #pragma unroll
         for (int c = 0; c < C; ++c)
 #pragma unroll
-          for (int k = 0; k < n; ++k) uc[c][k] = su[c * T::NTL + cb + k * T::CS];
+          for (int k = 0; k < n; ++k) uc[c][k] = su[c * T::NTLP + cb + k * T::CS];
         // vertex material at t: mA + mB t, from its end-face values
         float mA[T::KP], mB[T::KP];
         {
@@ -441,16 +479,16 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
           float g[C][DIM];
 #pragma unroll
           for (int c = 0; c < C; ++c) {
-            const float *rx = su + c * T::NTL + (cb - ci) + k * T::CS;
+            const float *rx = su + c * T::NTLP + (cb - ci) + k * T::CS;
             float s0 = 0.f;
 #pragma unroll
             for (int m = 0; m < n; ++m) s0 = fmaf(Dx[m], rx[m], s0);
             g[c][0] = s0;
             if (DIM == 3) {
-              const float *ry = su + c * T::NTL + (cb - T::LX * cj) + k * T::CS;
+              const float *ry = su + c * T::NTLP + (cb - T::LXP * cj) + k * T::CS;
               float s1 = 0.f;
 #pragma unroll
-              for (int m = 0; m < n; ++m) s1 = fmaf(Dy[m], ry[m * T::LX], s1);
+              for (int m = 0; m < n; ++m) s1 = fmaf(Dy[m], ry[m * T::LXP], s1);
               g[c][NL > 1 ? 1 : 0] = s1;
             }
             float sz = 0.f;
@@ -599,7 +637,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
 #pragma unroll
         for (int c = 0; c < C; ++c)
 #pragma unroll
-          for (int k = 0; k < n; ++k) sK[(e * C + c) * ND + pl + T::ES * k] = f[c][k];
+          for (int k = 0; k < n; ++k) sK[e * T::SKE + c * ND + pl + T::ES * k] = f[c][k];
       }
     }
     __syncthreads();
@@ -668,7 +706,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
         }
         if (valid && lo >= 0) {
 #pragma unroll
-          for (int c = 0; c < C; ++c) sK[(e * C + c) * ND + p] += sK[(lo * C + c) * ND + pn];
+          for (int c = 0; c < C; ++c) sK[e * T::SKE + c * ND + p] += sK[lo * T::SKE + c * ND + pn];
         }
       }
       __syncthreads();
@@ -678,7 +716,8 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
     // result, accumulation over colours, Newmark update at the last colour
     auto finish_slot = [&](int l, const SlotIn &in) {
       if (in.w == kSlotEmpty) return;
-      const int x = l % T::LX, y = (l / T::LX) % T::LY, z = DIM == 3 ? l / T::LXY : 0;
+      const int z = DIM == 3 ? l / T::LXYP : 0, rl = l - z * T::LXYP;
+      const int y = rl / T::LXP, x = rl - y * T::LXP;
       const int ea = min(x / N, ti.ex - 1), eb = min(y / N, ti.ey - 1);
       const int ez = DIM == 3 ? min(z / N, ti.ez - 1) : 0;
       const int e = ea + TX * (eb + TY * ez);
@@ -689,7 +728,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         const uint32_t idx = (s * C + c) * GG + g;
-        const float tot = sK[(e * C + c) * ND + p];
+        const float tot = sK[e * T::SKE + c * ND + p];
         const float t = (first && !has_ext) ? tot : in.prev[c] + tot;
         if (!fin) {
           gacc[idx] = t;
@@ -698,7 +737,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
         if (q3 && c == 2) continue;
         const float a = in.mi * t;
         float w = in.vw[c], kw = comp_v(in.kc[c]);
-        float u = su[c * T::NTL + l], ku = comp_u(in.kc[c]);
+        float u = su[c * T::NTLP + l], ku = comp_u(in.kc[c]);
         const int reps = (q3 && c == 0) ? 2 : 1;
         for (int rep = 0; rep < reps; ++rep) {
           kahan(w, kw, gdt * a);
@@ -727,7 +766,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
       for (int r = 0; r < KB; ++r) {
         const int k = tid + (r0 + r) * T::BT;
         ls[r] = (r0 + r < T::NIT && k < T::NTL)
-                    ? (!T::NODE_ORDER ? k : T::SLOT_LDS ? (int)ssl[k] : (int)slots[k])
+                    ? (!T::NODE_ORDER ? T::lds_slot(k) : T::SLOT_LDS ? (int)ssl[k] : T::lds_slot((int)slots[k]))
                     : -1;
         load_slot(ls[r], in[r], false);
       }
@@ -741,7 +780,7 @@ __global__ void __launch_bounds__((TS<DIM, N, PHYS, TX, TY, TZ, HO>::BT))
       const int l = sl(r);
       if (l >= 0) {
 #pragma unroll
-        for (int c = 0; c < C; ++c) su[c * T::NTL + l] = nu[r][c];
+        for (int c = 0; c < C; ++c) su[c * T::NTLP + l] = nu[r][c];
       }
     }
     __syncthreads();  // next lattice staged; sK and sF free again
