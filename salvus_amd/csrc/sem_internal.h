@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/salvus_sem.h"
+#include "tile_layout.h"
 
 namespace sem {
 
@@ -24,9 +25,6 @@ constexpr uint32_t kFlagFirst = 1u << 30;      // first colour to touch the node
 constexpr uint32_t kFlagLast = 1u << 31;       // last colour: finalise (mass + Newmark)
 // Tile lattice slot outside the mesh (partial tile at the mesh edge).
 constexpr uint32_t kSlotEmpty = 0xffffffffu;
-// Sources (over all shots of a group) one tile may hold; groups with more
-// in some tile take the element kernel.
-constexpr int kTileMaxSrc = 64;
 
 // ---------------------------------------------------------------- errors
 void set_error(const std::string &msg);
