@@ -302,18 +302,52 @@ static int build_plan(sem_batch *b) {
   // mode, tile shape); SEM_TILE_GENERAL=1 keeps the general tile kernel
   const char *gen = getenv("SEM_TILE_GENERAL");
   const bool general = gen && gen[0] == '1';
-  std::map<std::tuple<int, int, int, int, int, int>, std::vector<int>> keys;
-  for (int gi = 0; gi < (int)b->groups.size(); ++gi) {
+  // lanes: SEM_LANES=k (1..4) splits the groups over k streams (not with
+  // coupled pairs, whose two groups exchange loads every step)
+  const char *ln = getenv("SEM_LANES");
+  const int ngroups = (int)b->groups.size();
+  std::vector<int> lane(ngroups, 0);
+  b->lanes = ln ? std::max(1, std::min(atoi(ln), kMaxLanes)) : 1;
+  if (!b->couples.empty()) b->lanes = 1;
+  b->lanes = std::min(b->lanes, std::max(1, ngroups));
+  if (b->lanes > 1) {
+    // largest first onto the lightest lane (work: elements x shots)
+    std::vector<int> order(ngroups);
+    for (int gi = 0; gi < ngroups; ++gi) order[gi] = gi;
+    auto work = [&](int gi) { return (double)b->groups[gi].mesh->E * b->groups[gi].S; };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    std::vector<double> load(b->lanes, 0.0);
+    for (int gi : order) {
+      const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      lane[gi] = l;
+      load[l] += work(gi);
+    }
+    while ((int)b->lane_streams.size() < b->lanes - 1) {
+      hipStream_t st;
+      if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        set_error("stream creation failed");
+        return -2;
+      }
+      b->lane_streams.push_back(st);
+    }
+    while (b->lane_ev.size() < (size_t)kMaxBlockSteps * kMaxLanes) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      b->lane_ev.push_back(e);
+    }
+  }
+  std::map<std::tuple<int, int, int, int, int, int, int>, std::vector<int>> keys;
+  for (int gi = 0; gi < ngroups; ++gi) {
     sem_mesh *m = b->groups[gi].mesh;
     const bool t = b->groups[gi].tiled;
     const int shape = t ? m->tiles.T[0] * 10000 + m->tiles.T[1] * 100 + m->tiles.T[2] : 0;
     const int tk = t ? 1 + 2 * (int)(m->tile_diag && !general) : 0;
-    keys[std::make_tuple(tk, m->dim, m->order, m->physics, geo_mode(m), shape)].push_back(gi);
+    keys[std::make_tuple(lane[gi], tk, m->dim, m->order, m->physics, geo_mode(m), shape)].push_back(gi);
   }
   for (auto &l : b->launches) dfree(l.d_items);
   b->launches.clear();
   for (auto &kv : keys) {
-    const bool tile = std::get<0>(kv.first) != 0;
+    const bool tile = std::get<1>(kv.first) != 0;
     int maxcol = 0;
     for (int gi : kv.second) {
       sem_mesh *m = b->groups[gi].mesh;
@@ -351,20 +385,21 @@ static int build_plan(sem_batch *b) {
       if (items.empty()) continue;
       LaunchSet L;
       L.tile = tile;
-      L.diag = (std::get<0>(kv.first) & 2) != 0;
-      L.dim = std::get<1>(kv.first);
-      L.order = std::get<2>(kv.first);
-      L.physics = std::get<3>(kv.first);
-      L.gmode = std::get<4>(kv.first);
+      L.diag = (std::get<1>(kv.first) & 2) != 0;
+      L.dim = std::get<2>(kv.first);
+      L.order = std::get<3>(kv.first);
+      L.physics = std::get<4>(kv.first);
+      L.gmode = std::get<5>(kv.first);
       L.nd = L.dim == 2 ? (L.order + 1) * (L.order + 1) : (L.order + 1) * (L.order + 1) * (L.order + 1);
       L.c = L.physics == SEM_ACOUSTIC ? 1 : L.dim;
       L.colour = c;
       L.nitems = (int)items.size();
-      const int shape = std::get<5>(kv.first);
+      const int shape = std::get<6>(kv.first);
       L.T[0] = shape / 10000;
       L.T[1] = (shape / 100) % 100;
       L.T[2] = shape % 100;
       L.grid = 0;
+      L.lane = std::get<0>(kv.first);
       if (tile) {
         int per_cu = 1;
         SEM_TRY(tile_kernel_prepare(L.dim, L.order, L.physics, L.T, &per_cu));
@@ -449,11 +484,11 @@ int finalize_groups(sem_batch *b, int epb_hint) {
     int64_t min_steps = INT64_MAX;
     for (auto &g : b->groups) min_steps = std::min(min_steps, std::max<int64_t>(g.nsteps, 1));
     if (b->groups.empty()) min_steps = 1;
-    b->block_steps = (int)std::min<int64_t>(std::max<int64_t>(1, 4096 / std::max<int>(1, (int)b->launches.size())), 50);
+    b->block_steps = (int)std::min<int64_t>(std::max<int64_t>(1, 4096 / std::max<int>(1, (int)b->launches.size())), kMaxBlockSteps);
     b->block_steps = std::max(1, std::min<int>(b->block_steps, (int)min_steps));
     // SEM_GRAPH_STEPS=k: steps per captured graph block (1..50)
     const char *gs = getenv("SEM_GRAPH_STEPS");
-    if (gs && atoi(gs) > 0) b->block_steps = std::max(1, std::min(atoi(gs), 50));
+    if (gs && atoi(gs) > 0) b->block_steps = std::max(1, std::min(atoi(gs), kMaxBlockSteps));
   }
   HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
@@ -490,7 +525,16 @@ int enqueue_fused_step(sem_batch *b, int koff, int ku_only, int last) {
     b->ev_launches += (int64_t)b->launches.size();
     HIPCHK(hipEventRecord(e0, b->stream));
   }
+  // lanes: fork from the batch stream, launch, join back (events of this
+  // step's slot; a captured block holds at most kMaxBlockSteps steps)
+  const int nl = b->lanes;
+  hipEvent_t *lev = nl > 1 ? &b->lane_ev[(size_t)(koff % kMaxBlockSteps) * kMaxLanes] : nullptr;
+  if (nl > 1) {
+    HIPCHK(hipEventRecord(lev[0], b->stream));
+    for (int l = 1; l < nl; ++l) HIPCHK(hipStreamWaitEvent(b->lane_streams[l - 1], lev[0], 0));
+  }
   for (auto &L : b->launches) {
+    hipStream_t st = L.lane > 0 && L.lane < nl ? b->lane_streams[L.lane - 1] : b->stream;
     FusedArgs a;
     a.groups = b->d_groups;
     a.items = L.d_items;
@@ -501,10 +545,14 @@ int enqueue_fused_step(sem_batch *b, int koff, int ku_only, int last) {
     a.ap_koff = last;
     a.pad = 0;
     auto go = [&]() {
-      return L.tile ? launch_tile(L.dim, L.order, L.physics, L.T, L.diag, L.grid, a, b->stream)
-                    : launch_fused(L.dim, L.order, L.physics, L.gmode, L.nitems, b->epb, L.nd, L.c, a, b->stream);
+      return L.tile ? launch_tile(L.dim, L.order, L.physics, L.T, L.diag, L.grid, a, st)
+                    : launch_fused(L.dim, L.order, L.physics, L.gmode, L.nitems, b->epb, L.nd, L.c, a, st);
     };
     SEM_TRY(go());
+  }
+  for (int l = 1; l < nl; ++l) {
+    HIPCHK(hipEventRecord(lev[l], b->lane_streams[l - 1]));
+    HIPCHK(hipStreamWaitEvent(b->stream, lev[l], 0));
   }
   if (timed) HIPCHK(hipEventRecord(b->ev.back(), b->stream));
   if (ku_only) return 0;
@@ -683,6 +731,8 @@ int sem_batch_destroy(sem_batch *b) {
   (void)hipSetDevice(b->device);
   (void)hipStreamSynchronize(b->stream);
   batch_release(b);
+  for (auto e : b->lane_ev) (void)hipEventDestroy(e);
+  for (auto st : b->lane_streams) (void)hipStreamDestroy(st);
   (void)hipStreamDestroy(b->stream);
   delete b;
   return 0;

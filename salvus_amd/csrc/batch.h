@@ -10,6 +10,9 @@
 
 namespace sem {
 
+constexpr int kMaxBlockSteps = 50;  // steps per captured graph block
+constexpr int kMaxLanes = 4;        // streams one step's launches may use
+
 template <typename T>
 inline int dalloc(T **p, size_t count, size_t *bytes = nullptr) {
   *p = nullptr;
@@ -112,6 +115,7 @@ struct LaunchSet {
   int nitems;
   int T[3];  // tile shape (tile kernel)
   int grid;  // tile kernel: workgroups (one per tile)
+  int lane;  // stream lane (0: the batch stream, 1: the second stream)
   WorkItem *d_items;
 };
 
@@ -151,6 +155,12 @@ struct sem_batch {
   std::vector<hipEvent_t> ev;  // timing: event pairs around timed kernels
   int64_t ev_launches = 0;       // kernel launches the event pairs cover
   std::vector<sem::CouplePair> couples;  // coupled fluid-solid shot pairs
+  // Lanes: the groups split into parts of about equal work, each part's
+  // colour launches on its own stream (lane 0: `stream`), forked from and
+  // joined back into `stream` once per step.
+  int lanes = 1;
+  std::vector<hipStream_t> lane_streams;  // lanes 1.. (created on demand)
+  std::vector<hipEvent_t> lane_ev;        // per step of a block: fork, then a join per lane
 };
 
 namespace sem {

@@ -192,6 +192,40 @@ def test_batched_identical_to_alone():
         np.testing.assert_array_equal(A.get_field(a, "u"), B.get_field(k, "u"))
 
 
+@pytest.mark.parametrize("no_graph", ["0", "1"])
+def test_lanes_identical(monkeypatch, no_graph):
+    """Groups on two meshes split over two or four streams (SEM_LANES: each
+    part's colour launches on its own stream, joined once per step) equal
+    the one-stream schedule bit for bit, in graph replays and eagerly."""
+    P1 = pb.make(3, 1, 4, order=4, seed=17, nsteps=60)
+    P2 = pb.make(3, 1, 5, order=4, seed=18, nsteps=60)
+    M1, M2 = mesh_for(P1), mesh_for(P2)
+    monkeypatch.setenv("SEM_NO_GRAPH", no_graph)
+    monkeypatch.setenv("SEM_GRAPH_STEPS", "7")
+    out = []
+    for lanes in ("1", "2", "4"):
+        monkeypatch.setenv("SEM_LANES", lanes)
+        B = sem.Batch()
+        ids = []
+        for M, P in ((M1, P1), (M1, P1), (M2, P2), (M2, P2), (M2, P2)):
+            k = B.add_shot(M, P["nsteps"], P["dt"])
+            B.add_source(k, P["src"]["loc"] * (1.0 + 0.05 * len(ids)), P["src"]["amp"], P["src"]["f0"],
+                         P["src"]["t0"], P["src"]["dir"])
+            for r in P["recs"]:
+                B.add_receiver(k, r)
+            ids.append(k)
+        B.prepare(sem.PATH_FUSED, 0, 1)
+        assert B.config()["groups"] == 5
+        B.run(25)
+        B.run(35)
+        out.append([(B.seismograms(k), B.get_field(k, "u")) for k in ids])
+        B.close()
+    for other in out[1:]:
+        for (s1, u1), (s2, u2) in zip(out[0], other):
+            np.testing.assert_array_equal(s1, s2)
+            np.testing.assert_array_equal(u1, u2)
+
+
 @pytest.mark.parametrize("block", ["", "7"])
 def test_graph_replay_identical_to_eager(monkeypatch, block):
     """Runs of 40 steps: with the default block (50) each is one captured
