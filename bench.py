@@ -261,6 +261,9 @@ def main():
                      "tile_bytes_per_launch": tile_launch_bytes,
                      "tile_achieved_GBs": tile_achieved,
                      "tile_frac": tile_achieved / HBM_PEAK_GBS,
+                     "note": "achieved and frac price the element formulation's bytes (an "
+                             "equivalent bandwidth, which can exceed the peak); the tile kernel "
+                             "moves tile_bytes_per_launch, and HBM carries traffic per launch",
                      "avg_launch_us": avg_launch_s * 1e6,
                      "launches_per_step": launches_per_step,
                      "kernel_ms_per_step": kms / timing_steps},
