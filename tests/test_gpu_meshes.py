@@ -64,6 +64,57 @@ def test_hex_eigenfunction_reference_bound(order, bound, kernel, monkeypatch):
     assert err <= bound * 1.01  # test_Scalar_3D.cpp:213-232
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("kernel", ["tile", "element", "plain"])
+def test_elastic_standing_p_wave(dim, kernel, monkeypatch):
+    """u_x = sin(pi x/L) cos(omega t), Dirichlet on the x faces: exact for the
+    elastic operators when the coupling stiffness C13 (2-D) / c12 and c13
+    (3-D) vanishes, so the run stays within the start-up amplitude omega dt/2
+    of the closed form and keeps the other components at zero; and it equals
+    the FP64 oracle's run within the run tolerance.  On 4^3 hexes the tile
+    path takes the order-4 4x2x2 elastic tile, whose Dirichlet flags are read
+    through the padded LDS lattice."""
+    monkeypatch.setenv("SEM_NO_TILES", "1" if kernel == "element" else "0")
+    if dim == 3:
+        monkeypatch.setenv("SEM_TILE", "4,2,2")
+    Lb, order, n_el = 1000.0, 4, 4
+    if dim == 2:
+        rho, c11 = 2000.0, 2000.0 * 3000.0 ** 2
+        cells, coords = meshgen.box_quad(n_el, n_el, Lb, Lb)
+        par = np.tile([rho, c11, 0.0, 1.1 * c11, 0.3 * c11], 4 * len(cells))
+    else:
+        rho, vph = 2000.0, 3000.0
+        c11 = rho * vph ** 2
+        # c12 = c11 - 2 c66 = 0 (VSH^2 = VPH^2/2); c13 = ETA (c11 - 2 c44) = 0 (ETA = 0)
+        cells, coords = meshgen.box_hex(n_el, n_el, n_el, Lb, Lb, Lb)
+        par = np.tile([rho, 3300.0, vph, 1400.0, vph / np.sqrt(2), 0.0], 8 * len(cells))
+    M = sem.Mesh(dim, order, sem.ELASTIC, cells, coords, par)
+    if kernel == "tile":
+        assert M.ntiles > 0 and (dim == 2 or M.tile_shape == (4, 2, 2))
+    xyz = M.node_coords()
+    ux = np.sin(np.pi * xyz[:, 0] / Lb)
+    u0 = np.zeros((dim, M.G))
+    u0[0] = ux
+    wall = np.isclose(xyz[:, 0], 0, atol=1e-9 * Lb) | np.isclose(xyz[:, 0], Lb, atol=1e-9 * Lb)
+    M.set_dirichlet(wall)
+    omega, dt, nsteps = np.pi / Lb * np.sqrt(c11 / rho), 2e-4, 600
+    prob = sem.Order2Newmark(M, dt, nsteps, path=sem.PATH_PLAIN if kernel == "plain" else sem.PATH_FUSED)
+    prob.set_field("u", u0)
+    S = po.Sim(dim, order, po.ELASTIC, cells, coords, par)
+    S.set_field("u", u0)
+    S.set_dirichlet(wall)
+    worst = 0.0
+    for _ in range(nsteps // 50):
+        prob.takeTimeStep(50)
+        S.run(dt, 50)
+        u, ref = prob.get_field("u"), S.get_field("u")
+        worst = max(worst, np.abs(u[0] - np.cos(omega * prob.time()) * ux).max())
+        assert np.abs(u[1:]).max() < 1e-6
+        assert np.abs(u - ref).max() <= TOL_RUN * np.abs(ref).max()
+    print("dim %d %s: worst error %.3g (bound %.3g)" % (dim, kernel, worst, omega * dt / 2 * 1.05))
+    assert worst < omega * dt / 2 * 1.05
+
+
 # ---------------------------------------------------------------- rotated cells
 QUAD_SIGNS = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]])            # QuadP1.h:98-108
 HEX_SIGNS = np.array([[-1, -1, -1], [-1, 1, -1], [1, 1, -1], [1, -1, -1],

@@ -306,7 +306,8 @@ def test_tile_shapes(tile, monkeypatch):
 
 @pytest.mark.parametrize("dim,physics,order,tile", [
     (2, 0, 3, "8,8"), (2, 1, 5, "16,8"), (2, 0, 6, "16,8"), (3, 0, 2, "4,2,2"),
-    (3, 1, 5, "2,2,2"), (3, 0, 5, "4,2,2"), (3, 0, 4, "4,4,2")])
+    (3, 1, 5, "2,2,2"), (3, 0, 5, "4,2,2"), (3, 0, 4, "4,4,2"),
+    (3, 1, 4, "4,4,2"), (3, 1, 4, "4,2,2"), (3, 1, 4, "2,2,2")])
 def test_tile_orders(dim, physics, order, tile, monkeypatch):
     """Tile kernels of other orders and shapes agree with the oracle."""
     monkeypatch.setenv("SEM_TILE", tile)
