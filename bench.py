@@ -201,7 +201,8 @@ def main():
     # ---------------- batched path
     B, ids = make_batch(sem, meshes, shots, sem.PATH_FUSED, total, args.spg)
     cfg = B.config()
-    log("batched path prepared: %s" % cfg)
+    lanes = B.lanes()
+    log("batched path prepared: %s, %d lane(s)" % (cfg, lanes))
     B.run(args.warmup)
     t0 = time.perf_counter()
     B.run(args.steps)
@@ -248,7 +249,7 @@ def main():
                                              else "tile and element kernels"),
                    "tile_shapes": [list(M.tile_shape) if M.tile_shape else None for M in meshes],
                    "shots_per_group": cfg["shots_per_group"], "groups": cfg["groups"],
-                   "graph_block_steps": cfg["block_steps"]},
+                   "graph_block_steps": cfg["block_steps"], "lanes": lanes},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": ("k_tile<%d,4,%s,%s>" % (dim, ("acoustic", "elastic")[physics], shape)
@@ -266,6 +267,8 @@ def main():
                              "moves tile_bytes_per_launch, and HBM carries traffic per launch",
                      "avg_launch_us": avg_launch_s * 1e6,
                      "launches_per_step": launches_per_step,
+                     "launch_note": "a launch is one colour of one kernel shape: with several "
+                                    "lanes, its per-lane dispatches run side by side and count once",
                      "kernel_ms_per_step": kms / timing_steps},
     }
     if traffic is not None:

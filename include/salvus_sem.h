@@ -256,11 +256,20 @@ int sem_batch_couple(sem_batch *batch, sem_coupling *coupling, int fluid_shot, i
  * graph block steps, groups, groups on the tile kernel (the others take the
  * element kernel)}. */
 int sem_batch_config(const sem_batch *batch, int64_t *cfg);
+/* Streams (lanes) the batched path's launches use: the groups split into
+ * parts of about equal work, each part's colour launches on its own stream,
+ * forked from and joined into the batch stream once per step; results are
+ * the same bit for bit with any count.  1 by default; SEM_LANES=k (1..4)
+ * chooses (at most one per group; 1 with coupled pairs); 0 before
+ * sem_batch_prepare.  No counterpart in the reference (one problem
+ * per process). */
+int sem_batch_lanes(const sem_batch *batch, int *lanes);
 /* Timing of the dominant kernel during the last sem_batch_run when the
  * environment sets SEM_TIME_KERNELS=1 (steps then run without graphs): total
  * ms and number of launches, from HIP events on the batch stream (batched
  * path: one pair around each step's colour launches of the tile or element
- * kernel; plain path: one pair around each stiffness launch). */
+ * kernel, the launches of one colour in different lanes, which run side by
+ * side, counted once; plain path: one pair around each stiffness launch). */
 int sem_batch_kernel_time(const sem_batch *batch, double *ms, int64_t *launches);
 
 /* Reference time-loop semantics (Options.cpp:84-97, Main.cpp:45): number

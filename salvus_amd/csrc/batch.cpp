@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <set>
 #include <tuple>
 
 #include "batch.h"
@@ -307,7 +308,7 @@ static int build_plan(sem_batch *b) {
   const char *ln = getenv("SEM_LANES");
   const int ngroups = (int)b->groups.size();
   std::vector<int> lane(ngroups, 0);
-  b->lanes = ln ? std::max(1, std::min(atoi(ln), kMaxLanes)) : 1;
+  b->lanes = ln && atoi(ln) > 0 ? std::min(atoi(ln), kMaxLanes) : 1;
   if (!b->couples.empty()) b->lanes = 1;
   b->lanes = std::min(b->lanes, std::max(1, ngroups));
   if (b->lanes > 1) {
@@ -410,6 +411,13 @@ static int build_plan(sem_batch *b) {
       b->launches.push_back(L);
     }
   }
+  // colour launches of one step: launches of the same kernel and colour in
+  // different lanes run side by side and count once
+  std::set<std::tuple<int, int, int, int, int, int, int, int>> per_step;
+  for (const auto &L : b->launches)
+    per_step.insert(std::make_tuple((int)L.tile, (int)L.diag, L.dim, L.order, L.physics, L.gmode,
+                                    L.T[0] * 10000 + L.T[1] * 100 + L.T[2], L.colour));
+  b->step_launches = (int)per_step.size();
   HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
 }
@@ -522,7 +530,7 @@ int enqueue_fused_step(sem_batch *b, int koff, int ku_only, int last) {
     HIPCHK(hipEventCreate(&e1));
     b->ev.push_back(e0);
     b->ev.push_back(e1);
-    b->ev_launches += (int64_t)b->launches.size();
+    b->ev_launches += (int64_t)b->step_launches;
     HIPCHK(hipEventRecord(e0, b->stream));
   }
   // lanes: fork from the batch stream, launch, join back (events of this
@@ -1067,6 +1075,11 @@ int sem_batch_config(const sem_batch *b, int64_t *cfg) {
   int64_t tiled = 0;
   for (const auto &g : b->groups) tiled += g.tiled ? 1 : 0;
   cfg[4] = tiled;
+  return 0;
+}
+
+int sem_batch_lanes(const sem_batch *b, int *lanes) {
+  *lanes = b->prepared ? b->lanes : 0;
   return 0;
 }
 

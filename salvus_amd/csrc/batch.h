@@ -159,6 +159,7 @@ struct sem_batch {
   // colour launches on its own stream (lane 0: `stream`), forked from and
   // joined back into `stream` once per step.
   int lanes = 1;
+  int step_launches = 0;                  // colour launches per step, lanes side by side counted once
   std::vector<hipStream_t> lane_streams;  // lanes 1.. (created on demand)
   std::vector<hipEvent_t> lane_ev;        // per step of a block: fork, then a join per lane
 };

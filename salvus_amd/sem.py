@@ -35,7 +35,7 @@ EXPORTS = (
     "sem_batch_destroy", "sem_batch_add_shot", "sem_shot_add_source",
     "sem_shot_add_moment_source", "sem_shot_add_receiver", "sem_shot_set_field",
     "sem_shot_get_field", "sem_batch_prepare", "sem_batch_run", "sem_shot_progress",
-    "sem_shot_get_seismograms", "sem_batch_config", "sem_batch_kernel_time",
+    "sem_shot_get_seismograms", "sem_batch_config", "sem_batch_lanes", "sem_batch_kernel_time",
     "sem_reference_step_count", "sem_server_create", "sem_server_destroy", "sem_server_clock",
     "sem_server_submit", "sem_server_pump", "sem_server_request", "sem_server_result",
     "sem_mesh_autotune", "sem_shot_add_file_source", "sem_receivers_write_h5",
@@ -95,6 +95,7 @@ def lib():
         L.sem_shot_progress.argtypes = [_VP, C.c_int, _I64, _D]
         L.sem_shot_get_seismograms.argtypes = [_VP, C.c_int, _F]
         L.sem_batch_config.argtypes = [_VP, _I64]
+        L.sem_batch_lanes.argtypes = [_VP, C.POINTER(C.c_int)]
         L.sem_batch_kernel_time.argtypes = [_VP, _D, _I64]
         L.sem_reference_step_count.restype = C.c_int64
         L.sem_reference_step_count.argtypes = [C.c_double, C.c_double, _D]
@@ -503,6 +504,11 @@ class Batch:
         _check(lib().sem_batch_config(self._h, _l(c)))
         return dict(elems_per_block=int(c[0]), shots_per_group=int(c[1]),
                     block_steps=int(c[2]), groups=int(c[3]), tiled_groups=int(c[4]))
+
+    def lanes(self):
+        n = C.c_int(0)
+        _check(lib().sem_batch_lanes(self._h, C.byref(n)))
+        return n.value
 
     def kernel_time(self):
         ms, n = np.zeros(1), np.zeros(1, dtype=np.int64)
