@@ -102,8 +102,9 @@ def coupled_parameters(cells, coords, model_centres, variables):
     fluid = element_fluid(2, cells, coords, model_centres, variables)
     par_f = vertex_parameters(2, 0, cells[fluid], coords, model_centres, variables)
     par_s = vertex_parameters(2, 1, cells[~fluid], coords, model_centres, variables)
+    name = resolve_name("RHO", variables)  # the reference's error when absent
     _, nearest = cKDTree(np.asarray(model_centres, dtype=np.float64)).query(
         element_centres(cells[fluid], coords))
-    rho = np.stack([np.asarray(variables["RHO_%d" % v], dtype=np.float64)[nearest]
+    rho = np.stack([np.asarray(variables["%s_%d" % (name, v)], dtype=np.float64)[nearest]
                     for v in range(cells.shape[1])], axis=1)
     return fluid, par_f, par_s, rho

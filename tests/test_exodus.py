@@ -190,6 +190,10 @@ def test_mixed_model_flags_and_coupled_parameters(tmp_path):
         model.element_fluid(2, cells, coords, centres, dict(variables, fluid=np.full(len(cells), 2.0)))
     with pytest.raises(KeyError, match="Fluid flag"):
         model.element_fluid(2, cells, coords, centres, {k: a for k, a in variables.items() if k != "fluid"})
+    # no RHO for rho_0: the reference's missing-parameter error
+    no_rho = {k: a for k, a in variables.items() if not k.startswith("RHO_")}
+    with pytest.raises(KeyError, match="RHO"):
+        model.coupled_parameters(cells, coords, centres, no_rho)
     # hexahedra: all fluid or all elastic, never both
     hc, hx = meshgen.box_hex(2, 1, 1, 20.0, 10.0, 10.0)
     hcent = model.element_centres(hc, hx)
