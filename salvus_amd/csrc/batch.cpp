@@ -299,8 +299,9 @@ static void batch_release(sem_batch *b) {
 // Work items per (kernel key, colour): tiles of tiled groups, EPB-element
 // slices of the element kernel's colour ranges otherwise.
 static int build_plan(sem_batch *b) {
-  // key = (tile? | diagonal geometry << 1, dim, order, physics, geometry
-  // mode, tile shape); SEM_TILE_GENERAL=1 keeps the general tile kernel
+  // key = (lane, tile? | diagonal geometry << 1, dim, order, physics,
+  // geometry mode, tile shape); SEM_TILE_GENERAL=1 keeps the general tile
+  // kernel
   const char *gen = getenv("SEM_TILE_GENERAL");
   const bool general = gen && gen[0] == '1';
   // lanes: SEM_LANES=k (1..4) splits the groups over k streams (not with
