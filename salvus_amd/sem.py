@@ -386,6 +386,28 @@ class Coupling:
         return self._h
 
 
+def coupled_from_exodus(mesh_path, model_path=None, order=4, quirks=None, device=0):
+    """The 2-D fluid-solid problem of an Exodus mesh and model whose elemental
+    "fluid" flag assigns each element its physics, as the reference builds
+    its coupled elements (ExodusModel::getElementType, ExodusModel.cpp:473-518;
+    Mesh.cpp:314-333 and :480-521): returns (fluid Mesh, solid Mesh,
+    Coupling).  The model defaults to the mesh file."""
+    from salvus_amd import exodus, model
+    dim, cells, coords, _ = exodus.read_mesh(mesh_path)
+    if dim != 2:
+        raise SemError("coupled_from_exodus: the fluid-solid coupling is 2-D (quads)")
+    _, centres, variables = exodus.read_model(model_path or mesh_path)
+    fluid, vp, solid_par, rho = model.coupled_parameters(cells, coords, centres, variables)
+    if fluid.all() or not fluid.any():
+        raise SemError("coupled_from_exodus: the model marks every element %s"
+                       % ("fluid" if fluid.all() else "elastic"))
+    (cf, xf, vp), (cs, xs, solid_par) = split_coupled(cells, coords, fluid, (vp, solid_par))
+    q = DEFAULT_QUIRKS if quirks is None else quirks
+    mf = Mesh(2, order, ACOUSTIC, cf, xf, vp, q, device)
+    ms = Mesh(2, order, ELASTIC, cs, xs, solid_par, q, device)
+    return mf, ms, Coupling(mf, ms, rho)
+
+
 class Batch:
     """Shots advanced together (Problem::initializeElements source/receiver
     attach, Problem.cpp:34-142; the step loop of Main.cpp:45-66)."""

@@ -200,3 +200,19 @@ def test_mixed_model_flags_and_coupled_parameters(tmp_path):
     assert model.element_fluid(3, hc, hx, hcent, {"fluid": np.ones(2)}).all()
     with pytest.raises(ValueError, match="hexahedral"):
         model.element_fluid(3, hc, hx, hcent, {"fluid": np.array([1.0, 0.0])})
+
+
+def test_coupled_from_exodus_refusals(tmp_path):
+    """sem.coupled_from_exodus refuses, before any device work, a hexahedral
+    file (the coupling is 2-D) and a model whose flag marks every element
+    alike."""
+    hc, hx = meshgen.box_hex(2, 1, 1, 20.0, 10.0, 10.0)
+    hexes = tmp_path / "hex.e"
+    write_exodus(hexes, hx, [(hc[:, [0, 3, 2, 1, 4, 5, 6, 7]] + 1, "HEX8")], elem_vars={"fluid": np.ones(2)})
+    with pytest.raises(sem.SemError, match="2-D"):
+        sem.coupled_from_exodus(hexes)
+    cells, coords = meshgen.box_quad(3, 2, 30.0, 20.0)
+    allfluid = tmp_path / "fluid.e"
+    _mixed_model_file(allfluid, cells, coords, np.ones(len(cells), dtype=bool))
+    with pytest.raises(sem.SemError, match="every element fluid"):
+        sem.coupled_from_exodus(allfluid)

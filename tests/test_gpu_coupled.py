@@ -191,5 +191,19 @@ def test_mixed_exodus_model_coupled_run(tmp_path):
         err = np.abs(got - ref[key]).max() / np.abs(ref[key]).max()
         print("exodus mixed %s: max rel err %.2e" % (key, err), flush=True)
         assert err < TOL, (key, err)
+    # the same problem in one call from the file: bit for bit
+    mf2, ms2, cpl2 = sem.coupled_from_exodus(path)
+    assert (mf2.E, mf2.G, ms2.E, ms2.G) == (mf.E, mf.G, ms.E, ms.G)
+    B2 = sem.Batch()
+    kf2 = B2.add_shot(mf2, nsteps, dt)
+    ks2 = B2.add_shot(ms2, nsteps, dt)
+    B2.add_source(kf2, [12.0, 21.0], 1.0, f0, 1.2 / f0)
+    B2.couple(cpl2, kf2, ks2)
+    B2.prepare(sem.PATH_FUSED)
+    B2.run(nsteps)
+    for k, k2 in ((kf, kf2), (ks, ks2)):
+        np.testing.assert_array_equal(B2.get_field(k2, "u"), B.get_field(k, "u"))
+    B2.close()
+    cpl2.close()
     B.close()
     cpl.close()
